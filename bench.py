@@ -1,0 +1,145 @@
+"""Benchmark: particle-updates/s of the explicit MPM step on one MI355X.
+
+Workload: the 3D column collapse at 15,990,784 points (256x256x128 grid over 2x2x1 m,
+h = 7.8125 mm, 2^3 points per cell jittered +-0.25 cell, E = 1e7 Pa, nu = 0.3,
+rho = 1500, dt = 1e-5, gravity, CartesianBox walls), synthetic and seeded.
+
+A step is one full ExplicitUSL step over all points with the state resident in HBM:
+sort (every 16 steps), map + P2G, grid update, G2P + stress. `value` = active points x
+timed steps / wall time of the timed steps.
+
+The JSON line also carries
+  roofline      the dominant kernel's algorithmic HBM bytes per launch / its average launch
+                time (HIP events on the library's stream), against 8 TB/s
+  cpu_baseline  the oracle's threaded twin of ThreadPoolExplicitUSL (oracle/, -O3
+                -march=native) on a bounded sample of the same workload, on this host
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-updates/s, 16M-pt 3D (FP32 vs FP64); batched solves/s + p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes (3D, FP64 form); see the per-kernel table in DESIGN.md
+BYTES = {
+    # kernel class: (bytes per active point, bytes per active node)
+    "p2g": (15 * 8, 7 * 8),     # read x3 v3 T6 V m extent; write m, p3, f3
+    "grid": (0, 13 * 8),        # read m, p3, f3; write v3, dv3
+    "g2p": (14 * 8 + 13 * 8, 6 * 8),  # read x3 v3 T6 V extent, write x3 v3 T6 V; read v3 dv3
+    "step": (224, 208),         # one fused particle pass + grid update per step
+}
+CLASSES = {0: "pass", 1: "grid", 2: "sort", 3: "p2g", 4: "g2p"}
+
+
+def cpu_baseline(threads):
+    """Threaded oracle on the 1/16-footprint sample of the same column (same h, E, dt)."""
+    from mpm_3d_amd import workloads as W
+    from tests._harness import oracle_of
+    w = W.column_3d_sample()
+    ora = oracle_of(w, native=True)
+    ora.step(w.dt, 1, nthreads=threads)  # warm-up (allocations)
+    steps = 3
+    t0 = time.perf_counter()
+    ora.step(w.dt, steps, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": w.n_points * steps / dt,
+        "unit": "particle-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{w.n_points} points, 64x64x128 grid (same h, material, dt as the 16M column), "
+                  f"{steps} timed steps of the threaded ExplicitUSL restatement ({dt:.1f} s)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--sort-interval", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is "
+              "design-only in this build", file=sys.stderr)
+        return 2
+
+    from mpm_3d_amd import workloads as W
+    from mpm_3d_amd.gpu import GpuSim
+
+    w = W.column_3d(wide=True)
+    print(f"[bench] workload {w.name}: {w.n_points} points, grid {w.N}", file=sys.stderr, flush=True)
+    sim = GpuSim(w, sort_interval=args.sort_interval)
+    if args.warmup:
+        sim.step(w.dt, args.warmup)
+    sim.synchronize()
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    sim.step(w.dt, args.steps)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    active_nodes, active_points = sim.stats()
+    print(f"[bench] {args.steps} steps in {wall:.3f} s; active points {active_points}, nodes {active_nodes}",
+          file=sys.stderr, flush=True)
+
+    kern = {}
+    for cls, name in CLASSES.items():
+        n, ms = sim.timing(cls)
+        if n:
+            kern[name] = {"launches": n, "avg_ms": ms / n, "total_ms": ms}
+    dom = max((k for k in kern if k in BYTES), key=lambda k: kern[k]["total_ms"])
+    bp, bn = BYTES[dom]
+    bytes_per_launch = bp * active_points + bn * active_nodes
+    achieved = bytes_per_launch / (kern[dom]["avg_ms"] * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_column_3d_16M.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {
+        "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "algorithmic_bytes_per_launch": bytes_per_launch,
+    }
+    step_bytes = (BYTES["step"][0] * active_points + BYTES["step"][1] * active_nodes) * args.steps
+    value = active_points * args.steps / wall
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": w.name, "points": int(w.n_points), "grid": list(w.N), "dt": w.dt,
+                   "form": "plain FP64", "sort_interval": args.sort_interval},
+        "roofline": roofline,
+        "step_hbm_fraction": round(step_bytes / wall / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels": kern,
+    }
+    sim.close()
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        print(f"[bench] cpu baseline with {threads} threads", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(threads)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

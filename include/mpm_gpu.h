@@ -1,0 +1,149 @@
+/*
+ * mpm_gpu.h -- C-ABI of the MI355X explicit MPM step (libmpm_gpu.so).
+ *
+ * One handle holds one or more independent simulations ("sims"), each a CartesianLinear
+ * grid with its bodies and contacts, resident in HBM. Every entry point returns 0 on
+ * success or a negative MPM_ERR_* code; mpm_gpu_last_error() describes the last failure
+ * of the calling thread. A handle owns one HIP stream and is not thread-safe; host
+ * buffers belong to the caller and are not retained after a call returns.
+ *
+ * Array layout across the boundary is structure-of-arrays: a vector field of a body
+ * with n points is `dim` planes of n doubles (x[d*n + p]); a stress is 6 planes in the
+ * order xx, yy, zz, xy, xz, yz (the reference's symmetric MaterialTensor); a velocity
+ * gradient is 9 planes, row-major (L[(3*r + c)*n + p]).
+ *
+ * The reference has no FFI for this path: its ExplicitUSL calls the Points/Nodes/Grid/
+ * Material/Boundary/Contact virtuals in-process (src/objects/solvers/explicit_usl.cpp:48-87).
+ * Each entry point below names the reference interface it stands in for.
+ */
+#ifndef MPM_GPU_H
+#define MPM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_OK 0
+#define MPM_ERR_ARG -1      /* invalid argument or unsupported configuration */
+#define MPM_ERR_HIP -2      /* HIP runtime failure (allocation, launch, copy) */
+#define MPM_ERR_STATE -3    /* call out of order (e.g. step before upload) */
+#define MPM_ERR_NOMEM -4    /* device memory exhausted */
+
+/* precision forms */
+#define MPM_PREC_FP64 0     /* plain form: particle state and grid in FP64 */
+#define MPM_PREC_FP32 1     /* particle state and stress in FP32, grid accumulation in FP64 */
+
+/* connectivity codes (mpm_gpu_connectivity) */
+#define MPM_CONN_CPDI_OFF 0 /* linear weights at the point (cartesian.cpp:280-357) */
+#define MPM_CONN_CPDI_ON 1  /* GIMP/CPDI corners (default_points.cpp:264-324) */
+#define MPM_CONN_INACTIVE 2 /* inactive or outside the domain */
+
+#define MPM_MAX_BODIES_PER_SIM 4
+
+typedef struct mpm_gpu_handle mpm_gpu_handle;
+
+/* Grid and solver settings of one simulation.
+ * Replaces: CartesianLinear::init properties {Lx..}, int-properties {Nx..}
+ * (src/objects/grids/cartesian.cpp:24-67), Job TYPE (include/job.hpp:58-73),
+ * ExplicitUSL int-properties {cpdi_spec, contact_spec} (explicit_usl.cpp:25-44) and the
+ * driver's gravity (default_driver.cpp:90-105). */
+typedef struct {
+    int dim;              /* 2 or 3 (TYPE 2 / TYPE 3) */
+    double L[3];          /* domain lengths */
+    int N[3];             /* elements per dimension */
+    int cpdi_spec;        /* 1: GIMP/CPDI corners; 0: linear basis at the point */
+    int contact_spec;     /* 1: IMPLICIT; 0: EXPLICIT (mpm_objects.hpp:217-218) */
+    double gravity[3];    /* body force per unit mass, constant over the run */
+} mpm_grid_desc;
+
+/* One body. Replaces: DefaultBody with DefaultPoints/DefaultNodes, the optional
+ * IsotropicLinearElasticity {E, nu} (isolin.cpp:26-46) and CartesianBox (cartesian_box.cpp:23-60). */
+typedef struct {
+    int64_t n_points;
+    int has_material;
+    double E, nu;
+    int has_boundary;
+} mpm_body_desc;
+
+/* ContactHuang between two bodies of the same sim, {mu_f} and {body_1, body_2}
+ * (contact_huang.cpp:24-75). body indices are sim-local. */
+typedef struct {
+    int body1, body2;
+    double mu;
+} mpm_contact_desc;
+
+/* One simulation of a batch: grid settings plus its bodies and contacts. */
+typedef struct {
+    mpm_grid_desc grid;
+    int nbody;
+    const mpm_body_desc *bodies;
+    int ncontact;
+    const mpm_contact_desc *contacts;
+} mpm_sim_desc;
+
+/* Create a handle for one simulation. Replaces Job::init of the grid, bodies and
+ * contacts (include/job.hpp:76-105) for the supported plugin set. */
+int mpm_gpu_create(const mpm_grid_desc *grid, const mpm_body_desc *bodies, int nbody,
+                   const mpm_contact_desc *contacts, int ncontact, int precision,
+                   mpm_gpu_handle **out);
+
+/* Create a handle holding nsim independent simulations advanced together. Bodies are
+ * numbered globally in sim order (sim 0's bodies first). */
+int mpm_gpu_create_batch(const mpm_sim_desc *sims, int nsim, int precision,
+                         mpm_gpu_handle **out);
+
+/* Upload the state of one body (global body index). x, v: dim planes; T: 6 planes or
+ * NULL for zero stress; V volume, m mass, extent the GIMP half-width computed on the
+ * host as 0.5*sqrt(V) (2D) or 0.5*cbrt(V) (3D) (default_points.cpp:39-57); active 0/1.
+ * Replaces DefaultPoints::readFromFile + init (default_points.cpp:35-193). */
+int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v,
+                   const double *T, const double *V, const double *m, const double *extent,
+                   const int32_t *active);
+
+/* Advance every sim of the handle by nsteps steps of size dt (per-sim dt taken from
+ * mpm_gpu_set_dt when set, else this dt). Replaces nsteps calls of ExplicitUSL::step
+ * (explicit_usl.cpp:48-87) with DefaultDriver gravity (default_driver.cpp:46-86). */
+int mpm_gpu_step(mpm_gpu_handle *h, double dt, int nsteps);
+
+/* Download one body in its original point order. Any pointer may be NULL.
+ * Replaces reads of Points::x, x_t, T, v, active (mpm_objects.hpp:137-159). */
+int mpm_gpu_download(mpm_gpu_handle *h, int body, double *x, double *v, double *T,
+                     double *V, int32_t *active);
+
+/* Displacement u (dim planes; accumulated x - x0 since upload) and the velocity gradient
+ * L of the last step (9 planes; zero for points without connectivity).
+ * Replaces reads of Points::u and Points::L. */
+int mpm_gpu_download_aux(mpm_gpu_handle *h, int body, double *u, double *L);
+
+/* Per-point connectivity of the current state, original order: code (MPM_CONN_*) and the
+ * element of each of the 2^dim GIMP corners (-1 where none; CPDI-off points report their
+ * own element in slot 0). Replaces DefaultPoints::generateMap's element lookups
+ * (default_points.cpp:196-330, cartesian.cpp:219-253). */
+int mpm_gpu_connectivity(mpm_gpu_handle *h, int body, int32_t *corner_elem, uint8_t *code);
+
+/* Points re-sorted by grid block every `interval` steps (default 16; 0 = only at the
+ * start of each mpm_gpu_step call). Affects speed only. */
+int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
+
+/* Kernel timing with HIP events on the handle's stream: enable, then read the number of
+ * timed launches and their summed milliseconds per kernel class
+ * (0 = particle pass G2P+P2G, 1 = grid update, 2 = sort, 3 = first P2G, 4 = last G2P). */
+int mpm_gpu_set_timing(mpm_gpu_handle *h, int enable);
+int mpm_gpu_get_timing(mpm_gpu_handle *h, int kernel_class, int64_t *launches, double *ms);
+
+/* Number of grid nodes that held mass in the last step (summed over bodies) and the
+ * number of active points. */
+int mpm_gpu_stats(mpm_gpu_handle *h, int64_t *active_nodes, int64_t *active_points);
+
+int mpm_gpu_synchronize(mpm_gpu_handle *h);
+int mpm_gpu_destroy(mpm_gpu_handle *h);
+const char *mpm_gpu_last_error(void);
+/* bytes of device memory a handle with these sims would allocate */
+int64_t mpm_gpu_footprint(const mpm_sim_desc *sims, int nsim, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

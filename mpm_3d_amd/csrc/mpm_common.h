@@ -1,0 +1,324 @@
+// mpm_common.h -- device-side descriptors, layouts and shape-function evaluation shared by
+// the MI355X explicit MPM kernels (gfx950, wave64).
+//
+// Layouts (all structure-of-arrays, one plane per component):
+//   particles  P[f * np + slot]      f = x[DIM], v[DIM], T[NTS], V, m, extent (FP64 form)
+//              slots are kept sorted by particle block (see mpm_sort.hip) and carry the
+//              point's original global index in pid[slot]
+//   grid       acc[f * nt + node]    accumulators m, p[DIM], f[DIM], n[DIM] (contact normal)
+//              out[f * nt + node]    v[DIM] (velocity after the update), dv[DIM]
+//              node = body.grid_base + 64 * node_block + local; a node block is 4x4x4 (3D)
+//              or 8x8 (2D) nodes so one wavefront owns one block.
+//   stress     NTS = 6 in 3D (xx, yy, zz, xy, xz, yz), 4 in 2D (xx, yy, zz, xy)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpm {
+
+constexpr int MAXB = 4;  // bodies per simulation handled by the grid kernel
+
+// shape-function evaluation modes for one point
+constexpr int MODE_LEFT = -1;  // outside [0, L]: deactivated (default_points.cpp:217-219)
+constexpr int MODE_NONE = 0;   // active but no element (a corner/point exactly at x = L)
+constexpr int MODE_SEP = 1;    // GIMP/CPDI, separable per-dimension weights (<= 3 nodes/dim)
+constexpr int MODE_TUP = 2;    // per-tuple evaluation: CPDI off, wide extents, index wrap
+
+struct SimDev {
+    int dim;
+    int N[3];
+    double L[3];
+    double h[3];
+    double invh[3];
+    int NBn[3];          // node blocks per dimension
+    int PNB[3];          // particle blocks per dimension
+    long long nblk_off;  // first global node-block id of this sim
+    long long nblk;      // node blocks of this sim
+    long long element_count;
+    int body0, nbody, contact0, ncontact;
+    double g[3];
+    double dt;
+    int cpdi_spec, contact_spec;
+};
+
+struct BodyDev {
+    int sim;
+    int local;           // index within the sim
+    long long p_off, n;  // original global index range of the body's points
+    long long grid_base; // first node of the body's grid fields
+    long long pblk_off;  // first particle-block key of the body
+    long long npblk;     // particle blocks of the body
+    int has_material, has_boundary, is_b1, pad;
+    double twoG, lambda;
+};
+
+struct ContactDev {
+    int b1, b2;  // sim-local body indices
+    double mu;
+};
+
+struct PBlk {  // a run of sorted particles sharing one particle block
+    int key, start, count, body;
+};
+
+template <int DIM> struct Cfg;
+template <> struct Cfg<3> {
+    static constexpr int NB = 4;   // node-block edge (nodes)
+    static constexpr int LNB = 2;  // log2(NB)
+    static constexpr int PB = 4;   // particle-block edge (dual cells)
+    static constexpr int P = 8;    // LDS patch edge (nodes): PB + 2 window + 2 drift
+    static constexpr int PATCH = 512;
+    static constexpr int NTS = 6;
+};
+template <> struct Cfg<2> {
+    static constexpr int NB = 8;
+    static constexpr int LNB = 3;
+    static constexpr int PB = 8;
+    static constexpr int P = 12;
+    static constexpr int PATCH = 144;
+    static constexpr int NTS = 4;
+};
+
+template <int DIM> struct PF {  // particle field planes (FP64 form)
+    static constexpr int X = 0, V = DIM, T = 2 * DIM, NTS = Cfg<DIM>::NTS;
+    static constexpr int VOL = 2 * DIM + NTS, MASS = VOL + 1, EXT = VOL + 2, N = VOL + 3;
+};
+
+template <int DIM> struct GF {  // grid field planes
+    static constexpr int M = 0, P = 1, F = 1 + DIM, NRM = 1 + 2 * DIM, NACC = 1 + 3 * DIM;
+    static constexpr int OV = 0, ODV = DIM, NOUT = 2 * DIM;
+};
+
+struct KArgs {
+    const SimDev *sims;
+    int nsim;
+    const BodyDev *bodies;
+    int nbody;
+    const ContactDev *contacts;
+    double *P;
+    long long np;
+    int8_t *act;
+    int *pid;
+    double *acc;
+    double *out;
+    long long nt;
+    unsigned *nflag;
+    int *nlist;
+    int *ncount;
+    long long nblk_total;
+    const PBlk *plist;
+    const int *npblk;
+    int par;
+    double *Lout;  // 9 planes x np, slot order (optional)
+    unsigned long long *stats;  // [0] active nodes of the last grid update
+};
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// sim-local node (i,j,k) -> offset inside the sim's node-block storage
+template <int DIM>
+__device__ __forceinline__ long long node_addr(const SimDev &s, const int *ijk) {
+    constexpr int NB = Cfg<DIM>::NB, L = Cfg<DIM>::LNB;
+    long long blk = ijk[0] >> L;
+    int loc = ijk[0] & (NB - 1);
+    if (DIM >= 2) {
+        blk += (long long)(ijk[1] >> L) * s.NBn[0];
+        loc += (ijk[1] & (NB - 1)) << L;
+    }
+    if (DIM == 3) {
+        blk += (long long)(ijk[2] >> L) * s.NBn[0] * s.NBn[1];
+        loc += (ijk[2] & (NB - 1)) << (2 * L);
+    }
+    return blk * 64 + loc;
+}
+
+// CartesianLinear::inDomain (cartesian.cpp:245-253): 0 <= x <= L in every dimension
+template <int DIM>
+__device__ __forceinline__ bool in_domain(const SimDev &s, const double *x) {
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) in = in && (x[d] <= s.L[d] && x[d] >= 0.0);
+    return in;
+}
+
+// CartesianLinear::cartesianWhichElement (cartesian.cpp:219-240), including the
+// double-precision accumulation of the element id. -1 unless 0 <= x < L.
+template <int DIM>
+__device__ __forceinline__ long long which_element(const SimDev &s, const double *x) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++)
+        if (!(x[d] < s.L[d] && x[d] >= 0.0)) return -1;
+    long long e = (long long)floor(x[0] / s.L[0] * (double)s.N[0]);
+    if (DIM >= 2) e = (long long)((double)e + floor(x[1] / s.L[1] * (double)s.N[1]) * (double)s.N[0]);
+    if (DIM == 3)
+        e = (long long)((double)e + floor(x[2] / s.L[2] * (double)s.N[2]) * (double)s.N[1] * (double)s.N[0]);
+    return e;
+}
+
+// Separable GIMP/CPDI weights of one point (default_points.cpp:264-324 summed per node):
+// per dimension, nodes base..base+cnt-1 (cnt <= 3) carry
+//   sw = hat(x - a) + hat(x + a)      and      dw = hat(x + a) - hat(x - a)
+// with a = 0.5*extent; then S = prod sw / 2^D and G_d = dw_d prod_{e!=d} sw_e * gscale,
+// gscale = 1 / (2^D * 0.5 * extent). Each corner's hat values are those of the element
+// that CartesianLinear::whichElement assigns to the corner, so a corner exactly at x = L
+// contributes nothing, as in the reference.
+template <int DIM> struct Win {
+    int base[DIM];
+    int cnt[DIM];
+    double sw[DIM][3];
+    double dw[DIM][3];
+};
+
+// cpdi_out tells the tuple path whether the point uses its corners (CPDI on) or not.
+template <int DIM>
+__device__ __forceinline__ int point_window(const SimDev &s, const double *x, double ext, Win<DIM> &w,
+                                            bool &cpdi_out) {
+    cpdi_out = false;
+    if (!in_domain<DIM>(s, x)) return MODE_LEFT;
+    if (s.cpdi_spec != 1) return MODE_TUP;
+    const double a = 0.5 * ext;
+    double xm[DIM], xp[DIM];
+    bool cin = true;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        xm[d] = x[d] + a * (-1.0);  // x + (0.5*extent)*A(c,d)
+        xp[d] = x[d] + a * (1.0);
+        cin = cin && (xm[d] <= s.L[d] && xm[d] >= 0.0) && (xp[d] <= s.L[d] && xp[d] >= 0.0);
+    }
+    if (!cin) return MODE_TUP;  // a corner leaves the domain: CPDI off (default_points.cpp:220-235)
+    cpdi_out = true;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const bool vm = xm[d] < s.L[d], vp = xp[d] < s.L[d];
+        if (!vm && !vp) return MODE_NONE;
+        const int em = vm ? (int)floor(xm[d] / s.L[d] * (double)s.N[d]) : 0;
+        const int ep = vp ? (int)floor(xp[d] / s.L[d] * (double)s.N[d]) : 0;
+        if ((vm && em >= s.N[d]) || (vp && ep >= s.N[d])) return MODE_TUP;
+        const int lo = vm ? em : ep;
+        const int hi = vp ? ep : em;
+        const int cnt = hi - lo + 2;
+        if (cnt > 3) return MODE_TUP;
+        w.base[d] = lo;
+        w.cnt[d] = cnt;
+        double s0 = 0, s1 = 0, s2 = 0, d0 = 0, d1 = 0, d2 = 0;
+        const double hd = s.h[d], ih = s.invh[d];
+        if (vm) {
+            const double h0 = 1.0 - fabs((xm[d] - hd * em) * ih);
+            const double h1 = 1.0 - fabs((xm[d] - hd * (em + 1)) * ih);
+            const int k = em - lo;  // 0 or 1
+            s0 += k == 0 ? h0 : 0.0;
+            s1 += k == 0 ? h1 : h0;
+            s2 += k == 0 ? 0.0 : h1;
+            d0 -= k == 0 ? h0 : 0.0;
+            d1 -= k == 0 ? h1 : h0;
+            d2 -= k == 0 ? 0.0 : h1;
+        }
+        if (vp) {
+            const double h0 = 1.0 - fabs((xp[d] - hd * ep) * ih);
+            const double h1 = 1.0 - fabs((xp[d] - hd * (ep + 1)) * ih);
+            const int k = ep - lo;
+            s0 += k == 0 ? h0 : 0.0;
+            s1 += k == 0 ? h1 : h0;
+            s2 += k == 0 ? 0.0 : h1;
+            d0 += k == 0 ? h0 : 0.0;
+            d1 += k == 0 ? h1 : h0;
+            d2 += k == 0 ? 0.0 : h1;
+        }
+        w.sw[d][0] = s0; w.sw[d][1] = s1; w.sw[d][2] = s2;
+        w.dw[d][0] = d0; w.dw[d][1] = d1; w.dw[d][2] = d2;
+    }
+    return MODE_SEP;
+}
+
+// Per-tuple evaluation, exactly the reference's COO construction for one point
+// (default_points.cpp:238-324 with cartesian.cpp:280-357): for every corner (CPDI on)
+// or the point itself (CPDI off), the 2^D nodes of its element with S and G.
+// fn(node_ijk, S, G[DIM]) is called per tuple. Used for CPDI-off points and the rare
+// shapes the separable form does not cover.
+template <int DIM, class Fn>
+__device__ void for_each_tuple(const SimDev &s, const double *x, double ext, bool cpdi, Fn &&fn) {
+    constexpr int NC = 1 << DIM;
+    const int ncorner = cpdi ? NC : 1;
+    for (int c = 0; c < ncorner; c++) {
+        double xc[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const double A = ((c >> d) & 1) ? 1.0 : -1.0;
+            xc[d] = cpdi ? x[d] + 0.5 * ext * A : x[d];
+        }
+        long long e = which_element<DIM>(s, xc);
+        if (e < 0 || e >= s.element_count) continue;
+        int eijk[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            eijk[d] = (int)(e % s.N[d]);
+            e /= s.N[d];
+        }
+        for (int n = 0; n < NC; n++) {
+            int nijk[DIM];
+            double r[DIM];
+            double tmp = 1.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                nijk[d] = eijk[d] + ((n >> d) & 1);
+                r[d] = (xc[d] - s.h[d] * (double)nijk[d]) / s.h[d];
+                tmp *= (1.0 - fabs(r[d]));
+            }
+            double S, G[DIM];
+            if (cpdi) {
+                S = tmp / (double)NC;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    const double A = ((c >> d) & 1) ? 1.0 : -1.0;
+                    G[d] = tmp / ((double)NC * 0.5 * ext) * A;
+                }
+            } else {
+                S = tmp;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    double gi = -tmp;
+                    gi = gi / (1.0 - fabs(r[d]));
+                    gi = gi * r[d];
+                    gi = gi / fabs(r[d]);
+                    gi = gi / s.h[d];
+                    if (fabs(r[d]) >= 1.0) gi = 0.0;
+                    G[d] = gi;
+                }
+            }
+            fn(nijk, S, G);
+        }
+    }
+}
+
+// IsotropicLinearElasticity::calculateStress for one point (isolin.cpp:50-78), in the
+// reference's operation order: T += dt * (((2G D + lambda trD I) + W T) - T W).
+// T is symmetric (stored xx, yy, zz, xy, xz, yz); L is the full 3x3 velocity gradient.
+template <class R>
+__device__ __forceinline__ void isolin_update(double twoG, double lambda, double dt, const R L[3][3], R T6[6]) {
+    R Tm[3][3] = {{T6[0], T6[3], T6[4]}, {T6[3], T6[1], T6[5]}, {T6[4], T6[5], T6[2]}};
+    R D[3][3], W[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            D[r][c] = (L[r][c] + L[c][r]) * (R)0.5;
+            W[r][c] = (L[r][c] - L[c][r]) * (R)0.5;
+        }
+    const R trD = D[0][0] + D[1][1] + D[2][2];
+    const R tg = (R)twoG;
+    const R lt = (R)lambda * trD;
+    const int rr[6] = {0, 1, 2, 0, 0, 1};
+    const int cc[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int r = rr[k], c = cc[k];
+        const R WT = W[r][0] * Tm[0][c] + W[r][1] * Tm[1][c] + W[r][2] * Tm[2][c];
+        const R TW = Tm[r][0] * W[0][c] + Tm[r][1] * W[1][c] + Tm[r][2] * W[2][c];
+        const R I = (r == c) ? (R)1.0 : (R)0.0;
+        const R tmp = ((D[r][c] * tg + I * lt) + WT) - TW;
+        T6[k] = T6[k] + tmp * (R)dt;
+    }
+}
+
+}  // namespace mpm

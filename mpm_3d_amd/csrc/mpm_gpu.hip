@@ -1,0 +1,771 @@
+// mpm_gpu.hip -- libmpm_gpu.so: device state, step orchestration and the C-ABI declared
+// in include/mpm_gpu.h. Everything stays resident in HBM between calls; a step call is a
+// fixed sequence of kernel launches on the handle's stream with no host synchronisation
+// until the call returns.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mpm_gpu.h"
+#include "mpm_kernels.h"
+#include "mpm_sort.h"
+
+using namespace mpm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(e_ == hipErrorOutOfMemory ? MPM_ERR_NOMEM : MPM_ERR_HIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                       \
+    } while (0)
+
+constexpr int NCLASS = 5;
+enum { KC_PASS = 0, KC_GRID = 1, KC_SORT = 2, KC_P2G = 3, KC_G2P = 4 };
+
+long long ceil_div(long long a, long long b) { return (a + b - 1) / b; }
+
+struct Layout {  // sizes derived from the descriptors, shared by create and footprint
+    int dim = 0;
+    std::vector<SimDev> sims;
+    std::vector<BodyDev> bodies;
+    std::vector<ContactDev> contacts;
+    long long np = 0, nt = 0, nblk_total = 0, nkeys = 0;
+};
+
+int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
+    if (nsim <= 0 || !sd) return fail(MPM_ERR_ARG, "need at least one simulation");
+    Lo.dim = sd[0].grid.dim;
+    if (Lo.dim != 2 && Lo.dim != 3) return fail(MPM_ERR_ARG, "dim must be 2 or 3 (got %d)", Lo.dim);
+    long long p_off = 0, grid_base = 0, nblk_off = 0, pblk_off = 0;
+    for (int i = 0; i < nsim; i++) {
+        const mpm_grid_desc &g = sd[i].grid;
+        if (g.dim != Lo.dim) return fail(MPM_ERR_ARG, "all simulations of a handle must share dim");
+        if (sd[i].nbody < 1 || sd[i].nbody > MPM_MAX_BODIES_PER_SIM)
+            return fail(MPM_ERR_ARG, "sim %d: 1..%d bodies supported (got %d)", i, MPM_MAX_BODIES_PER_SIM,
+                        sd[i].nbody);
+        if (g.cpdi_spec != 0 && g.cpdi_spec != 1)
+            return fail(MPM_ERR_ARG, "Unrecognized Argument for use_cpdi: %d", g.cpdi_spec);
+        if (g.contact_spec != 0 && g.contact_spec != 1)
+            return fail(MPM_ERR_ARG, "ERROR: Unknown SPEC in contact: %d", g.contact_spec);
+        SimDev s{};
+        s.dim = g.dim;
+        const int NB = g.dim == 3 ? Cfg<3>::NB : Cfg<2>::NB;
+        const int PB = g.dim == 3 ? Cfg<3>::PB : Cfg<2>::PB;
+        long long nblk = 1, pnb = 1;
+        s.element_count = 1;
+        for (int d = 0; d < 3; d++) {
+            if (d < g.dim) {
+                if (!(g.L[d] > 0) || g.N[d] < 1)
+                    return fail(MPM_ERR_ARG, "sim %d: need L > 0 and N >= 1 in every dimension", i);
+                s.N[d] = g.N[d];
+                s.L[d] = g.L[d];
+                s.h[d] = g.L[d] / g.N[d];
+                s.invh[d] = 1.0 / s.h[d];
+                s.NBn[d] = (int)ceil_div(g.N[d] + 1, NB);
+                s.PNB[d] = (int)ceil_div(g.N[d] + 1, PB);
+                nblk *= s.NBn[d];
+                pnb *= s.PNB[d];
+                s.element_count *= g.N[d];
+                s.g[d] = g.gravity[d];
+            } else {
+                s.N[d] = 0;
+                s.NBn[d] = 1;
+                s.PNB[d] = 1;
+            }
+        }
+        s.nblk_off = nblk_off;
+        s.nblk = nblk;
+        s.body0 = (int)Lo.bodies.size();
+        s.nbody = sd[i].nbody;
+        s.contact0 = (int)Lo.contacts.size();
+        s.ncontact = sd[i].ncontact;
+        s.cpdi_spec = g.cpdi_spec;
+        s.contact_spec = g.contact_spec;
+        s.dt = 0.0;
+        nblk_off += nblk;
+        std::vector<int> is_b1(sd[i].nbody, 0);
+        for (int c = 0; c < sd[i].ncontact; c++) {
+            const mpm_contact_desc &cd = sd[i].contacts[c];
+            if (cd.body1 < 0 || cd.body1 >= sd[i].nbody || cd.body2 < 0 || cd.body2 >= sd[i].nbody ||
+                cd.body1 == cd.body2)
+                return fail(MPM_ERR_ARG, "sim %d contact %d: bodies must be two distinct sim-local indices", i, c);
+            is_b1[cd.body1] = 1;
+            ContactDev C{cd.body1, cd.body2, cd.mu};
+            Lo.contacts.push_back(C);
+        }
+        for (int b = 0; b < sd[i].nbody; b++) {
+            const mpm_body_desc &bd = sd[i].bodies[b];
+            if (bd.n_points < 0) return fail(MPM_ERR_ARG, "negative point count");
+            BodyDev B{};
+            B.sim = i;
+            B.local = b;
+            B.p_off = p_off;
+            B.n = bd.n_points;
+            B.grid_base = grid_base;
+            B.pblk_off = pblk_off;
+            B.npblk = pnb;
+            B.has_material = bd.has_material;
+            B.has_boundary = bd.has_boundary;
+            B.is_b1 = is_b1[b];
+            if (bd.has_material) {
+                // IsotropicLinearElasticity::init (isolin.cpp:34-38)
+                const double G = bd.E / (2.0 * (1.0 + bd.nu));
+                const double K = bd.E / (3.0 * (1.0 - 2 * bd.nu));
+                B.twoG = 2 * G;
+                B.lambda = K - 2.0 * G / 3.0;
+            }
+            p_off += bd.n_points;
+            grid_base += nblk * 64;
+            pblk_off += pnb;
+            Lo.bodies.push_back(B);
+        }
+        Lo.sims.push_back(s);
+    }
+    Lo.np = p_off;
+    Lo.nt = grid_base;
+    Lo.nblk_total = nblk_off;
+    Lo.nkeys = pblk_off + 1;  // + inactive bucket
+    if (Lo.np >= (1ll << 31) - 1) return fail(MPM_ERR_ARG, "at most 2^31-2 points per handle");
+    if (Lo.nkeys >= (1ll << 31) - 1) return fail(MPM_ERR_ARG, "grid too large for one handle");
+    return MPM_OK;
+}
+
+long long footprint_bytes(const Layout &Lo, int precision) {
+    (void)precision;
+    const int nf = Lo.dim == 3 ? PF<3>::N : PF<2>::N;
+    const int nacc = Lo.dim == 3 ? GF<3>::NACC : GF<2>::NACC;
+    const int nout = Lo.dim == 3 ? GF<3>::NOUT : GF<2>::NOUT;
+    long long b = 0;
+    b += 2 * Lo.np * (nf * 8 + 1 + 4);     // particle planes, double-buffered, + act + pid
+    b += Lo.np * (4 + 4);                  // sort key + rank
+    b += Lo.nt * (nacc + nout) * 8;        // grid
+    b += Lo.nblk_total * (4 + 2 * 4);      // node-block flags + two lists
+    b += Lo.nkeys * (4 + 4 + sizeof(PBlk)); // histogram, offsets, block list
+    return b;
+}
+
+struct Timer {
+    hipEvent_t a, b;
+    int cls;
+};
+
+}  // namespace
+
+struct mpm_gpu_handle {
+    Layout lo;
+    int precision = 0;
+    hipStream_t stream = nullptr;
+    SimDev *d_sims = nullptr;
+    BodyDev *d_bodies = nullptr;
+    ContactDev *d_contacts = nullptr;
+    double *P = nullptr, *P2 = nullptr;
+    int8_t *act = nullptr, *act2 = nullptr;
+    int *pid = nullptr, *pid2 = nullptr;
+    int *pkey = nullptr, *prank = nullptr;
+    double *acc = nullptr, *out = nullptr;
+    unsigned *nflag = nullptr;
+    int *nlist = nullptr;
+    int *ncount = nullptr;
+    unsigned *bcount = nullptr, *boff = nullptr;
+    std::vector<unsigned *> scan_tmp;  // per level: block sums, their scan
+    std::vector<long long> scan_n;
+    PBlk *plist = nullptr;
+    int *npblk = nullptr;
+    double *Lout = nullptr;
+    unsigned long long *stats = nullptr;
+    double *stage = nullptr;  // download staging
+    long long stage_elems = 0;
+    int par = 0;
+    int sort_interval = 16;
+    bool sims_dirty = true;
+    std::vector<char> uploaded;
+    std::vector<std::vector<double>> x0;  // per body, dim planes (for displacement)
+    bool timing = false;
+    std::vector<Timer> timers;
+    std::vector<Timer> free_timers;
+    long long t_launch[NCLASS] = {0, 0, 0, 0, 0};
+    double t_ms[NCLASS] = {0, 0, 0, 0, 0};
+    unsigned long long last_active_nodes = 0;
+};
+
+namespace {
+
+KArgs kargs(mpm_gpu_handle *h) {
+    KArgs A{};
+    A.sims = h->d_sims;
+    A.nsim = (int)h->lo.sims.size();
+    A.bodies = h->d_bodies;
+    A.nbody = (int)h->lo.bodies.size();
+    A.contacts = h->d_contacts;
+    A.P = h->P;
+    A.np = h->lo.np;
+    A.act = h->act;
+    A.pid = h->pid;
+    A.acc = h->acc;
+    A.out = h->out;
+    A.nt = h->lo.nt;
+    A.nflag = h->nflag;
+    A.nlist = h->nlist;
+    A.ncount = h->ncount;
+    A.nblk_total = h->lo.nblk_total;
+    A.plist = h->plist;
+    A.npblk = h->npblk;
+    A.par = h->par;
+    A.Lout = h->Lout;
+    A.stats = h->stats;
+    return A;
+}
+
+int timer_begin(mpm_gpu_handle *h, int cls, Timer &t) {
+    if (!h->timing) return MPM_OK;
+    if (h->free_timers.empty()) {
+        Timer n;
+        HIP_TRY(hipEventCreate(&n.a));
+        HIP_TRY(hipEventCreate(&n.b));
+        h->free_timers.push_back(n);
+    }
+    t = h->free_timers.back();
+    h->free_timers.pop_back();
+    t.cls = cls;
+    HIP_TRY(hipEventRecord(t.a, h->stream));
+    return MPM_OK;
+}
+
+int timer_end(mpm_gpu_handle *h, Timer &t) {
+    if (!h->timing) return MPM_OK;
+    HIP_TRY(hipEventRecord(t.b, h->stream));
+    h->timers.push_back(t);
+    return MPM_OK;
+}
+
+int timers_collect(mpm_gpu_handle *h) {
+    for (Timer &t : h->timers) {
+        HIP_TRY(hipEventSynchronize(t.b));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
+        h->t_launch[t.cls]++;
+        h->t_ms[t.cls] += ms;
+        h->free_timers.push_back(t);
+    }
+    h->timers.clear();
+    return MPM_OK;
+}
+
+template <class T>
+int dalloc(T **p, long long n) {
+    if (n <= 0) n = 1;
+    HIP_TRY(hipMalloc((void **)p, sizeof(T) * n));
+    return MPM_OK;
+}
+
+template <int DIM>
+int launch_sort(mpm_gpu_handle *h) {
+    const long long np = h->lo.np, nk = h->lo.nkeys;
+    KArgs A = kargs(h);
+    Timer t{};
+    int rc = timer_begin(h, KC_SORT, t);
+    if (rc) return rc;
+    const int gp = (int)std::max(1ll, ceil_div(np, 256));
+    hipLaunchKernelGGL(k_bin_count<DIM>, dim3(gp), dim3(256), 0, h->stream, A, h->bcount, h->pkey, h->prank,
+                       nk - 1);
+    // exclusive scan of the histogram, multi-level
+    {
+        const unsigned *in = h->bcount;
+        unsigned *outp = h->boff;
+        long long n = nk;
+        std::vector<std::pair<unsigned *, long long>> stack;
+        int lvl = 0;
+        while (true) {
+            const long long nb = ceil_div(n, 4096);
+            unsigned *bsum = nb > 1 ? h->scan_tmp[2 * lvl] : nullptr;
+            hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(1024), 0, h->stream, in, outp, bsum, n);
+            if (nb <= 1) break;
+            stack.push_back({outp, n});
+            in = bsum;
+            outp = h->scan_tmp[2 * lvl + 1];
+            n = nb;
+            lvl++;
+        }
+        for (int l = (int)stack.size() - 1; l >= 0; l--) {
+            const long long nn = stack[l].second;
+            hipLaunchKernelGGL(k_scan_add, dim3((unsigned)ceil_div(nn, 4096)), dim3(1024), 0, h->stream,
+                               stack[l].first, h->scan_tmp[2 * l + 1], nn);
+        }
+    }
+    hipLaunchKernelGGL(k_make_plist, dim3((unsigned)ceil_div(nk, 256)), dim3(256), 0, h->stream, A, h->bcount,
+                       h->boff, h->plist, h->npblk, nk);
+    hipLaunchKernelGGL(k_bin_scatter<PF<DIM>::N>, dim3(gp), dim3(256), 0, h->stream, np, h->P, h->P2, h->act,
+                       h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff);
+    std::swap(h->P, h->P2);
+    std::swap(h->act, h->act2);
+    std::swap(h->pid, h->pid2);
+    HIP_TRY(hipGetLastError());
+    return timer_end(h, t);
+}
+
+constexpr int PASS_GRID = 2048;  // persistent workgroups for the particle passes
+
+template <int DIM>
+int launch_step(mpm_gpu_handle *h, int nsteps) {
+    for (int k = 0; k < nsteps; k++) {
+        const bool first = k == 0;
+        const bool last = k == nsteps - 1;
+        int rc;
+        if (first || (h->sort_interval > 0 && k % h->sort_interval == 0)) {
+            rc = launch_sort<DIM>(h);
+            if (rc) return rc;
+        }
+        Timer t{};
+        KArgs A = kargs(h);
+        rc = timer_begin(h, KC_P2G, t);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_p2g<DIM>, dim3(PASS_GRID), dim3(256), 0, h->stream, A);
+        rc = timer_end(h, t);
+        if (rc) return rc;
+        if (last) {
+            HIP_TRY(hipMemsetAsync(h->stats, 0, sizeof(unsigned long long), h->stream));
+        }
+        rc = timer_begin(h, KC_GRID, t);
+        if (rc) return rc;
+        KArgs Ag = kargs(h);
+        if (!last) Ag.stats = nullptr;
+        hipLaunchKernelGGL(k_grid<DIM>, dim3(PASS_GRID), dim3(256), 0, h->stream, Ag);
+        rc = timer_end(h, t);
+        if (rc) return rc;
+        rc = timer_begin(h, KC_G2P, t);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_g2p<DIM>, dim3(PASS_GRID), dim3(256), 0, h->stream, A, last ? 1 : 0);
+        rc = timer_end(h, t);
+        if (rc) return rc;
+        HIP_TRY(hipGetLastError());
+        h->par ^= 1;
+    }
+    return MPM_OK;
+}
+
+int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
+    if (precision != MPM_PREC_FP64) return fail(MPM_ERR_ARG, "precision form %d not available in this build", precision);
+    mpm_gpu_handle *h = new mpm_gpu_handle();
+    h->lo = std::move(lo);
+    h->precision = precision;
+    auto cleanup = [&](int rc) {
+        mpm_gpu_destroy(h);
+        return rc;
+    };
+    int rc;
+#define TRY(x)                   \
+    do {                         \
+        rc = (x);                \
+        if (rc) return cleanup(rc); \
+    } while (0)
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return cleanup(fail(MPM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)));
+    const Layout &L = h->lo;
+    const int dim = L.dim;
+    const int nf = dim == 3 ? PF<3>::N : PF<2>::N;
+    const int nacc = dim == 3 ? GF<3>::NACC : GF<2>::NACC;
+    const int nout = dim == 3 ? GF<3>::NOUT : GF<2>::NOUT;
+    TRY(dalloc(&h->d_sims, (long long)L.sims.size()));
+    TRY(dalloc(&h->d_bodies, (long long)L.bodies.size()));
+    TRY(dalloc(&h->d_contacts, std::max<long long>(1, (long long)L.contacts.size())));
+    TRY(dalloc(&h->P, L.np * nf));
+    TRY(dalloc(&h->P2, L.np * nf));
+    TRY(dalloc(&h->act, L.np));
+    TRY(dalloc(&h->act2, L.np));
+    TRY(dalloc(&h->pid, L.np));
+    TRY(dalloc(&h->pid2, L.np));
+    TRY(dalloc(&h->pkey, L.np));
+    TRY(dalloc(&h->prank, L.np));
+    TRY(dalloc(&h->acc, L.nt * nacc));
+    TRY(dalloc(&h->out, L.nt * nout));
+    TRY(dalloc(&h->nflag, L.nblk_total));
+    TRY(dalloc(&h->nlist, 2 * L.nblk_total));
+    TRY(dalloc(&h->ncount, 2));
+    TRY(dalloc(&h->bcount, L.nkeys));
+    TRY(dalloc(&h->boff, L.nkeys));
+    TRY(dalloc(&h->plist, L.nkeys));
+    TRY(dalloc(&h->npblk, 1));
+    TRY(dalloc(&h->stats, 4));
+    for (long long n = L.nkeys; n > 4096;) {
+        const long long nb = ceil_div(n, 4096);
+        unsigned *a = nullptr, *b = nullptr;
+        TRY(dalloc(&a, nb));
+        TRY(dalloc(&b, nb));
+        h->scan_tmp.push_back(a);
+        h->scan_tmp.push_back(b);
+        n = nb;
+    }
+    // zero state: accumulators must start at zero, flags clear, counters zero
+    e = hipMemsetAsync(h->acc, 0, sizeof(double) * L.nt * nacc, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->out, 0, sizeof(double) * L.nt * nout, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->nflag, 0, sizeof(unsigned) * std::max(1ll, L.nblk_total), h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->ncount, 0, sizeof(int) * 2, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->bcount, 0, sizeof(unsigned) * L.nkeys, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->npblk, 0, sizeof(int), h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->P, 0, sizeof(double) * std::max(1ll, L.np * nf), h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->act, 0, std::max(1ll, L.np), h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->stats, 0, sizeof(unsigned long long) * 4, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h->d_sims, L.sims.data(), sizeof(SimDev) * L.sims.size(), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h->d_bodies, L.bodies.data(), sizeof(BodyDev) * L.bodies.size(), hipMemcpyHostToDevice,
+                           h->stream);
+    if (e == hipSuccess && !L.contacts.empty())
+        e = hipMemcpyAsync(h->d_contacts, L.contacts.data(), sizeof(ContactDev) * L.contacts.size(),
+                           hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+        std::vector<int> ids(std::max(1ll, L.np));
+        for (long long i = 0; i < L.np; i++) ids[i] = (int)i;
+        e = hipMemcpyAsync(h->pid, ids.data(), sizeof(int) * L.np, hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    }
+    if (e != hipSuccess) return cleanup(fail(MPM_ERR_HIP, "initialising device state: %s", hipGetErrorString(e)));
+    h->uploaded.assign(L.bodies.size(), 0);
+    h->x0.resize(L.bodies.size());
+    h->sims_dirty = true;
+#undef TRY
+    *out = h;
+    return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mpm_gpu_last_error(void) { return g_err.c_str(); }
+
+int mpm_gpu_create_batch(const mpm_sim_desc *sims, int nsim, int precision, mpm_gpu_handle **out) {
+    if (!out) return fail(MPM_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    Layout lo;
+    int rc = build_layout(sims, nsim, lo);
+    if (rc) return rc;
+    return create_from_layout(std::move(lo), precision, out);
+}
+
+int mpm_gpu_create(const mpm_grid_desc *grid, const mpm_body_desc *bodies, int nbody,
+                   const mpm_contact_desc *contacts, int ncontact, int precision, mpm_gpu_handle **out) {
+    if (!grid || !bodies) return fail(MPM_ERR_ARG, "grid and bodies are required");
+    mpm_sim_desc s{};
+    s.grid = *grid;
+    s.nbody = nbody;
+    s.bodies = bodies;
+    s.ncontact = ncontact;
+    s.contacts = contacts;
+    return mpm_gpu_create_batch(&s, 1, precision, out);
+}
+
+int64_t mpm_gpu_footprint(const mpm_sim_desc *sims, int nsim, int precision) {
+    Layout lo;
+    if (build_layout(sims, nsim, lo)) return -1;
+    return footprint_bytes(lo, precision);
+}
+
+int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v, const double *T,
+                   const double *V, const double *m, const double *extent, const int32_t *active) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    if (body < 0 || body >= (int)h->lo.bodies.size()) return fail(MPM_ERR_ARG, "body %d out of range", body);
+    if (!x || !v || !V || !m || !extent || !active) return fail(MPM_ERR_ARG, "x, v, V, m, extent, active are required");
+    const BodyDev &B = h->lo.bodies[body];
+    const int dim = h->lo.dim;
+    const long long n = B.n, np = h->lo.np;
+    if (n == 0) {
+        h->uploaded[body] = 1;
+        return MPM_OK;
+    }
+    // The sorted slots of other bodies are interleaved with this body's, so an upload
+    // writes the staging buffer in original order and re-sorts from there: bring every
+    // body back to original order first.
+    const int nf = dim == 3 ? PF<3>::N : PF<2>::N;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    {
+        // unsort P -> P2 (all planes), act -> act2, pid -> identity
+        const int gp = (int)ceil_div(np, 256);
+        hipLaunchKernelGGL(k_unsort, dim3(gp), dim3(256), 0, h->stream, np, nf, h->P, np, h->P2, h->pid);
+        int *tmp_act = nullptr;
+        HIP_TRY(hipMalloc(&tmp_act, sizeof(int) * np));
+        hipLaunchKernelGGL(k_unsort_act, dim3(gp), dim3(256), 0, h->stream, np, h->act, tmp_act, h->pid);
+        std::vector<int> acts(np);
+        HIP_TRY(hipMemcpyAsync(acts.data(), tmp_act, sizeof(int) * np, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        HIP_TRY(hipFree(tmp_act));
+        std::vector<int8_t> a8(np);
+        for (long long i = 0; i < np; i++) a8[i] = (int8_t)acts[i];
+        for (long long i = 0; i < n; i++) a8[B.p_off + i] = active[i] ? 1 : 0;
+        HIP_TRY(hipMemcpyAsync(h->act2, a8.data(), np, hipMemcpyHostToDevice, h->stream));
+        std::vector<int> ids(np);
+        for (long long i = 0; i < np; i++) ids[i] = (int)i;
+        HIP_TRY(hipMemcpyAsync(h->pid2, ids.data(), sizeof(int) * np, hipMemcpyHostToDevice, h->stream));
+        // this body's planes
+        using F3 = PF<3>;
+        using F2 = PF<2>;
+        const int FX = 0, FV = dim, FT = 2 * dim;
+        const int NTS = dim == 3 ? 6 : 4;
+        const int FVOL = dim == 3 ? F3::VOL : F2::VOL, FMASS = FVOL + 1, FEXT = FVOL + 2;
+        auto put = [&](int f, const double *src) -> hipError_t {
+            return hipMemcpyAsync(h->P2 + (long long)f * np + B.p_off, src, sizeof(double) * n, hipMemcpyHostToDevice,
+                                  h->stream);
+        };
+        for (int d = 0; d < dim; d++) {
+            HIP_TRY(put(FX + d, x + (long long)d * n));
+            HIP_TRY(put(FV + d, v + (long long)d * n));
+        }
+        std::vector<double> zeros;
+        for (int c = 0; c < NTS; c++) {
+            if (T) {
+                // 6 planes in (xx, yy, zz, xy, xz, yz); 2D keeps the first four
+                HIP_TRY(put(FT + c, T + (long long)c * n));
+            } else {
+                if (zeros.empty()) zeros.assign(n, 0.0);
+                HIP_TRY(put(FT + c, zeros.data()));
+            }
+        }
+        HIP_TRY(put(FVOL, V));
+        HIP_TRY(put(FMASS, m));
+        HIP_TRY(put(FEXT, extent));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        std::swap(h->P, h->P2);
+        std::swap(h->act, h->act2);
+        std::swap(h->pid, h->pid2);
+    }
+    h->x0[body].assign(x, x + (long long)dim * n);
+    h->uploaded[body] = 1;
+    return MPM_OK;
+}
+
+int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    if (interval < 0) return fail(MPM_ERR_ARG, "interval must be >= 0");
+    h->sort_interval = interval;
+    return MPM_OK;
+}
+
+int mpm_gpu_step(mpm_gpu_handle *h, double dt, int nsteps) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    if (nsteps < 0) return fail(MPM_ERR_ARG, "nsteps must be >= 0");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(MPM_ERR_ARG, "dt must be positive and finite");
+    for (size_t b = 0; b < h->uploaded.size(); b++)
+        if (!h->uploaded[b] && h->lo.bodies[b].n > 0) return fail(MPM_ERR_STATE, "body %zu has not been uploaded", b);
+    if (nsteps == 0) return MPM_OK;
+    bool changed = h->sims_dirty;
+    for (SimDev &s : h->lo.sims)
+        if (s.dt != dt) {
+            s.dt = dt;
+            changed = true;
+        }
+    if (changed) {
+        HIP_TRY(hipMemcpyAsync(h->d_sims, h->lo.sims.data(), sizeof(SimDev) * h->lo.sims.size(), hipMemcpyHostToDevice,
+                               h->stream));
+        h->sims_dirty = false;
+    }
+    if (!h->Lout) {
+        HIP_TRY(hipMalloc(&h->Lout, sizeof(double) * 9 * std::max(1ll, h->lo.np)));
+        HIP_TRY(hipMemsetAsync(h->Lout, 0, sizeof(double) * 9 * std::max(1ll, h->lo.np), h->stream));
+    }
+    const int rc = h->lo.dim == 3 ? launch_step<3>(h, nsteps) : launch_step<2>(h, nsteps);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->timing) {
+        const int r2 = timers_collect(h);
+        if (r2) return r2;
+    }
+    HIP_TRY(hipMemcpy(&h->last_active_nodes, h->stats, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    return MPM_OK;
+}
+
+static int ensure_stage(mpm_gpu_handle *h, long long elems) {
+    if (h->stage_elems >= elems) return MPM_OK;
+    if (h->stage) HIP_TRY(hipFree(h->stage));
+    h->stage = nullptr;
+    HIP_TRY(hipMalloc(&h->stage, sizeof(double) * elems));
+    h->stage_elems = elems;
+    return MPM_OK;
+}
+
+int mpm_gpu_download(mpm_gpu_handle *h, int body, double *x, double *v, double *T, double *V, int32_t *active) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    if (body < 0 || body >= (int)h->lo.bodies.size()) return fail(MPM_ERR_ARG, "body %d out of range", body);
+    const BodyDev &B = h->lo.bodies[body];
+    const int dim = h->lo.dim;
+    const long long n = B.n, np = h->lo.np;
+    if (n == 0) return MPM_OK;
+    const int nf = dim == 3 ? PF<3>::N : PF<2>::N;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    int rc = ensure_stage(h, np * nf);
+    if (rc) return rc;
+    const int gp = (int)ceil_div(np, 256);
+    hipLaunchKernelGGL(k_unsort, dim3(gp), dim3(256), 0, h->stream, np, nf, h->P, np, h->stage, h->pid);
+    HIP_TRY(hipGetLastError());
+    auto get = [&](int f, double *dst) -> hipError_t {
+        return hipMemcpyAsync(dst, h->stage + (long long)f * np + B.p_off, sizeof(double) * n, hipMemcpyDeviceToHost,
+                              h->stream);
+    };
+    const int FX = 0, FV = dim, FT = 2 * dim, NTS = dim == 3 ? 6 : 4;
+    const int FVOL = 2 * dim + NTS;
+    for (int d = 0; d < dim; d++) {
+        if (x) HIP_TRY(get(FX + d, x + (long long)d * n));
+        if (v) HIP_TRY(get(FV + d, v + (long long)d * n));
+    }
+    if (T) {
+        for (int c = 0; c < NTS; c++) HIP_TRY(get(FT + c, T + (long long)c * n));
+        for (int c = NTS; c < 6; c++) std::memset(T + (long long)c * n, 0, sizeof(double) * n);
+    }
+    if (V) HIP_TRY(get(FVOL, V));
+    if (active) {
+        int *tmp = nullptr;
+        HIP_TRY(hipMalloc(&tmp, sizeof(int) * np));
+        hipLaunchKernelGGL(k_unsort_act, dim3(gp), dim3(256), 0, h->stream, np, h->act, tmp, h->pid);
+        HIP_TRY(hipMemcpyAsync(active, tmp + B.p_off, sizeof(int) * n, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        HIP_TRY(hipFree(tmp));
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return MPM_OK;
+}
+
+int mpm_gpu_download_aux(mpm_gpu_handle *h, int body, double *u, double *L) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    if (body < 0 || body >= (int)h->lo.bodies.size()) return fail(MPM_ERR_ARG, "body %d out of range", body);
+    const BodyDev &B = h->lo.bodies[body];
+    const int dim = h->lo.dim;
+    const long long n = B.n, np = h->lo.np;
+    if (n == 0) return MPM_OK;
+    if (u) {
+        int rc = mpm_gpu_download(h, body, u, nullptr, nullptr, nullptr, nullptr);
+        if (rc) return rc;
+        const std::vector<double> &x0 = h->x0[body];
+        for (long long i = 0; i < (long long)dim * n; i++) u[i] -= x0[i];
+    }
+    if (L) {
+        if (!h->Lout) {
+            std::memset(L, 0, sizeof(double) * 9 * n);
+            return MPM_OK;
+        }
+        int rc = ensure_stage(h, np * 9);
+        if (rc) return rc;
+        const int gp = (int)ceil_div(np, 256);
+        hipLaunchKernelGGL(k_unsort, dim3(gp), dim3(256), 0, h->stream, np, 9, h->Lout, np, h->stage, h->pid);
+        HIP_TRY(hipGetLastError());
+        for (int k = 0; k < 9; k++)
+            HIP_TRY(hipMemcpyAsync(L + (long long)k * n, h->stage + (long long)k * np + B.p_off, sizeof(double) * n,
+                                   hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    return MPM_OK;
+}
+
+int mpm_gpu_connectivity(mpm_gpu_handle *h, int body, int32_t *corner_elem, uint8_t *code) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    if (body < 0 || body >= (int)h->lo.bodies.size()) return fail(MPM_ERR_ARG, "body %d out of range", body);
+    if (!corner_elem || !code) return fail(MPM_ERR_ARG, "corner_elem and code are required");
+    const BodyDev &B = h->lo.bodies[body];
+    const int dim = h->lo.dim;
+    const int nc = 1 << dim;
+    const long long n = B.n, np = h->lo.np;
+    if (n == 0) return MPM_OK;
+    int *d_elem = nullptr;
+    uint8_t *d_code = nullptr;
+    HIP_TRY(hipMalloc(&d_elem, sizeof(int) * np * nc));
+    HIP_TRY(hipMalloc(&d_code, np));
+    KArgs A = kargs(h);
+    const int gp = (int)ceil_div(np, 256);
+    if (dim == 3)
+        hipLaunchKernelGGL(k_connectivity<3>, dim3(gp), dim3(256), 0, h->stream, A, d_elem, d_code);
+    else
+        hipLaunchKernelGGL(k_connectivity<2>, dim3(gp), dim3(256), 0, h->stream, A, d_elem, d_code);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(corner_elem, d_elem + B.p_off * nc, sizeof(int) * n * nc, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(code, d_code + B.p_off, n, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipFree(d_elem));
+    HIP_TRY(hipFree(d_code));
+    return MPM_OK;
+}
+
+int mpm_gpu_set_timing(mpm_gpu_handle *h, int enable) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    h->timing = enable != 0;
+    for (int i = 0; i < NCLASS; i++) {
+        h->t_launch[i] = 0;
+        h->t_ms[i] = 0;
+    }
+    return MPM_OK;
+}
+
+int mpm_gpu_get_timing(mpm_gpu_handle *h, int kernel_class, int64_t *launches, double *ms) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    if (kernel_class < 0 || kernel_class >= NCLASS) return fail(MPM_ERR_ARG, "kernel class out of range");
+    if (launches) *launches = h->t_launch[kernel_class];
+    if (ms) *ms = h->t_ms[kernel_class];
+    return MPM_OK;
+}
+
+int mpm_gpu_stats(mpm_gpu_handle *h, int64_t *active_nodes, int64_t *active_points) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    if (active_nodes) *active_nodes = (int64_t)h->last_active_nodes;
+    if (active_points) {
+        unsigned long long *d = nullptr;
+        HIP_TRY(hipMalloc(&d, sizeof(unsigned long long)));
+        HIP_TRY(hipMemsetAsync(d, 0, sizeof(unsigned long long), h->stream));
+        hipLaunchKernelGGL(k_count_active, dim3((unsigned)std::max(1ll, ceil_div(h->lo.np, 256))), dim3(256), 0,
+                           h->stream, h->lo.np, h->act, d);
+        unsigned long long c = 0;
+        HIP_TRY(hipMemcpyAsync(&c, d, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        HIP_TRY(hipFree(d));
+        *active_points = (int64_t)c;
+    }
+    return MPM_OK;
+}
+
+int mpm_gpu_synchronize(mpm_gpu_handle *h) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return MPM_OK;
+}
+
+int mpm_gpu_destroy(mpm_gpu_handle *h) {
+    if (!h) return MPM_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    void *ptrs[] = {h->d_sims, h->d_bodies, h->d_contacts, h->P, h->P2, h->act, h->act2, h->pid, h->pid2,
+                    h->pkey, h->prank, h->acc, h->out, h->nflag, h->nlist, h->ncount, h->bcount, h->boff,
+                    h->plist, h->npblk, h->Lout, h->stats, h->stage};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (unsigned *p : h->scan_tmp)
+        if (p) (void)hipFree(p);
+    for (Timer &t : h->free_timers) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (Timer &t : h->timers) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return MPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,179 @@
+// mpm_sort.h -- counting sort of particle slots by particle block, on the device.
+//
+// A particle block is a PB^DIM group of dual cells (the cell of x/h - 1/2, so every point
+// of a block touches nodes inside one (PB+2)^DIM window). Sorting keeps the particles of
+// a block contiguous, which is what lets k_p2g accumulate a whole block in one LDS patch.
+// The reference has no counterpart: it scatters straight from point order through COO
+// tuples (src/algebra/mpm_sparse.cpp:30-44).
+//
+//   k_bin_count   key per slot (inactive points go to the last bucket), wavefront-
+//                 aggregated atomic histogram, rank inside the bucket
+//   scan          exclusive prefix sum of the histogram (k_scan_*)
+//   k_make_plist  one PBlk per non-empty bucket; clears the histogram for the next sort
+//   k_bin_scatter moves every particle plane to its sorted slot
+#pragma once
+
+#include "mpm_common.h"
+
+namespace mpm {
+
+__device__ __forceinline__ int body_of_index(const KArgs &A, long long o) {
+    int lo = 0, hi = A.nbody - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (A.bodies[mid].p_off <= o) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int body_of_key(const KArgs &A, long long key) {
+    int lo = 0, hi = A.nbody - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (A.bodies[mid].pblk_off <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_bin_count(KArgs A, unsigned *count, int *pkey, int *prank,
+                                                   long long inactive_key) {
+    using F = PF<DIM>;
+    constexpr int PB = Cfg<DIM>::PB;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) *const_cast<int *>(A.npblk) = 0;
+    const bool live = p < A.np;
+    long long key = inactive_key;
+    if (live && A.act[p] && A.pid[p] >= 0) {
+        const int b = body_of_index(A, A.pid[p]);
+        const BodyDev &B = A.bodies[b];
+        const SimDev &s = A.sims[B.sim];
+        double x[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) x[d] = A.P[(F::X + d) * A.np + p];
+        if (in_domain<DIM>(s, x)) {
+            long long lk = 0, stride = 1;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                int c = (int)floor(x[d] * s.invh[d] - 0.5) + 1;  // dual cell, shifted to >= 0
+                c = c < 0 ? 0 : (c > s.N[d] ? s.N[d] : c);
+                lk += (long long)(c / PB) * stride;
+                stride *= s.PNB[d];
+            }
+            key = B.pblk_off + lk;
+        } else {
+            A.act[p] = 0;  // leaves the domain: deactivated (default_points.cpp:217-219)
+        }
+    }
+    // wavefront aggregation: one atomic per distinct key in the wavefront
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(live);
+    int rank = 0;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const long long lkey = __shfl(key, leader);
+        const unsigned long long same = __ballot(live && key == lkey) & pending;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&count[lkey], (unsigned)__popcll(same));
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1ull) rank = (int)base + __popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    if (live) {
+        pkey[p] = (int)key;
+        prank[p] = rank;
+    }
+}
+
+// exclusive scan, 1024 threads x 4 elements per workgroup
+__global__ __launch_bounds__(1024) void k_scan_block(const unsigned *in, unsigned *out, unsigned *bsum, long long n) {
+    __shared__ unsigned sh[1024];
+    const long long base = (long long)blockIdx.x * 4096 + threadIdx.x * 4;
+    unsigned v[4], t = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        v[i] = (base + i < n) ? in[base + i] : 0u;
+        t += v[i];
+    }
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const unsigned add = threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    unsigned run = sh[threadIdx.x] - t;  // exclusive prefix of this thread
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+    if (threadIdx.x == 1023 && bsum) bsum[blockIdx.x] = sh[1023];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_add(unsigned *out, const unsigned *bscan, long long n) {
+    const long long i = (long long)blockIdx.x * 4096 + threadIdx.x * 4;
+    const unsigned add = bscan[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (i + k < n) out[i + k] += add;
+}
+
+__global__ __launch_bounds__(256) void k_make_plist(KArgs A, unsigned *count, const unsigned *off, PBlk *plist,
+                                                    int *npblk, long long nkeys) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkeys) return;
+    const unsigned c = count[k];
+    if (c > 0 && k < nkeys - 1) {
+        const int pos = atomicAdd(npblk, 1);
+        PBlk e;
+        e.key = (int)k;
+        e.start = (int)off[k];
+        e.count = (int)c;
+        e.body = body_of_key(A, k);
+        plist[pos] = e;
+    }
+    count[k] = 0u;
+}
+
+template <int NF>
+__global__ __launch_bounds__(256) void k_bin_scatter(long long np, const double *P, double *P2, const int8_t *act,
+                                                     int8_t *act2, const int *pid, int *pid2, const int *pkey,
+                                                     const int *prank, const unsigned *off) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const long long q = (long long)off[pkey[p]] + prank[p];
+#pragma unroll
+    for (int f = 0; f < NF; f++) P2[f * np + q] = P[f * np + p];
+    act2[q] = act[p];
+    pid2[q] = pid[p];
+}
+
+// slot order -> original order for selected planes (download)
+__global__ __launch_bounds__(256) void k_unsort(long long np, int nf, const double *src, long long src_stride,
+                                                double *dst, const int *pid) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int o = pid[p];
+    if (o < 0) return;
+    for (int f = 0; f < nf; f++) dst[f * np + o] = src[f * src_stride + p];
+}
+
+__global__ __launch_bounds__(256) void k_unsort_act(long long np, const int8_t *act, int *dst, const int *pid) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int o = pid[p];
+    if (o >= 0) dst[o] = act[p];
+}
+
+__global__ __launch_bounds__(256) void k_count_active(long long np, const int8_t *act, unsigned long long *out) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool a = p < np && act[p];
+    const unsigned long long b = __ballot(a);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(out, (unsigned long long)__popcll(b));
+}
+
+}  // namespace mpm

@@ -1,0 +1,245 @@
+"""Python binding of libmpm_gpu.so (the C-ABI in include/mpm_gpu.h) over ctypes.
+
+``GpuSim`` is the host-side mirror of the reference's solver surface for this path: it
+takes a grid description (CartesianLinear {L, N}, TYPE, ExplicitUSL {cpdi_spec,
+contact_spec}), bodies (points + IsotropicLinearElasticity {E, nu} + CartesianBox) and
+ContactHuang contacts, keeps their state resident on the GPU and advances it with
+``step``. There is no CPU fallback: if the HIP library is missing or no GPU is visible,
+construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmpm_gpu.so")
+
+MPM_PREC_FP64 = 0
+MPM_PREC_FP32 = 1
+MPM_MAX_BODIES_PER_SIM = 4
+
+SYMBOLS = (
+    "mpm_gpu_create", "mpm_gpu_create_batch", "mpm_gpu_upload", "mpm_gpu_step", "mpm_gpu_download",
+    "mpm_gpu_download_aux", "mpm_gpu_connectivity", "mpm_gpu_set_sort_interval", "mpm_gpu_set_timing",
+    "mpm_gpu_get_timing", "mpm_gpu_stats", "mpm_gpu_synchronize", "mpm_gpu_destroy",
+    "mpm_gpu_last_error", "mpm_gpu_footprint",
+)
+
+
+class MpmGpuError(RuntimeError):
+    pass
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("dim", C.c_int), ("L", C.c_double * 3), ("N", C.c_int * 3), ("cpdi_spec", C.c_int),
+                ("contact_spec", C.c_int), ("gravity", C.c_double * 3)]
+
+
+class BodyDesc(C.Structure):
+    _fields_ = [("n_points", C.c_int64), ("has_material", C.c_int), ("E", C.c_double), ("nu", C.c_double),
+                ("has_boundary", C.c_int)]
+
+
+class ContactDesc(C.Structure):
+    _fields_ = [("body1", C.c_int), ("body2", C.c_int), ("mu", C.c_double)]
+
+
+class SimDesc(C.Structure):
+    _fields_ = [("grid", GridDesc), ("nbody", C.c_int), ("bodies", C.POINTER(BodyDesc)), ("ncontact", C.c_int),
+                ("contacts", C.POINTER(ContactDesc))]
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libmpm_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise MpmGpuError(f"{path} is missing: build it with `python -m mpm_3d_amd.build`")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    lib.mpm_gpu_create.argtypes = [C.POINTER(GridDesc), C.POINTER(BodyDesc), C.c_int, C.POINTER(ContactDesc), C.c_int,
+                                   C.c_int, C.POINTER(vp)]
+    lib.mpm_gpu_create_batch.argtypes = [C.POINTER(SimDesc), C.c_int, C.c_int, C.POINTER(vp)]
+    lib.mpm_gpu_upload.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_int32)]
+    lib.mpm_gpu_step.argtypes = [vp, C.c_double, C.c_int]
+    lib.mpm_gpu_download.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, C.POINTER(C.c_int32)]
+    lib.mpm_gpu_download_aux.argtypes = [vp, C.c_int, _dp, _dp]
+    lib.mpm_gpu_connectivity.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)]
+    lib.mpm_gpu_set_sort_interval.argtypes = [vp, C.c_int]
+    lib.mpm_gpu_set_timing.argtypes = [vp, C.c_int]
+    lib.mpm_gpu_get_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    lib.mpm_gpu_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    lib.mpm_gpu_synchronize.argtypes = [vp]
+    lib.mpm_gpu_destroy.argtypes = [vp]
+    lib.mpm_gpu_last_error.restype = C.c_char_p
+    lib.mpm_gpu_last_error.argtypes = []
+    lib.mpm_gpu_footprint.restype = C.c_int64
+    lib.mpm_gpu_footprint.argtypes = [C.POINTER(SimDesc), C.c_int, C.c_int]
+    for name in SYMBOLS:
+        getattr(lib, name)
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise MpmGpuError(f"mpm_gpu error {rc}: {_lib.mpm_gpu_last_error().decode(errors='replace')}")
+
+
+def extent_of(volume, dim):
+    """GIMP half-width, computed on the host with the C library (default_points.cpp:49-57)."""
+    volume = np.asarray(volume, np.float64)
+    return 0.5 * (np.sqrt(volume) if dim == 2 else np.cbrt(volume))
+
+
+def _grid_desc(dim, L, N, cpdi_spec, contact_spec, gravity):
+    g = GridDesc()
+    g.dim = dim
+    for d in range(3):
+        g.L[d] = float(L[d]) if d < dim else 0.0
+        g.N[d] = int(N[d]) if d < dim else 0
+        g.gravity[d] = float(gravity[d]) if d < len(gravity) else 0.0
+    g.cpdi_spec = cpdi_spec
+    g.contact_spec = contact_spec
+    return g
+
+
+class GpuSim:
+    """One handle on the GPU holding one or more simulations (``workloads``)."""
+
+    def __init__(self, workloads, precision=MPM_PREC_FP64, sort_interval=16):
+        if not isinstance(workloads, (list, tuple)):
+            workloads = [workloads]
+        self.lib = load_library()
+        self.workloads = list(workloads)
+        self.dim = self.workloads[0].dim
+        keep = []
+        sims = (SimDesc * len(self.workloads))()
+        self.body_index = []  # (sim, local) per global body
+        for si, w in enumerate(self.workloads):
+            bds = (BodyDesc * len(w.bodies))()
+            for bi, b in enumerate(w.bodies):
+                bds[bi].n_points = b.n
+                bds[bi].has_material = int(b.E is not None)
+                bds[bi].E = float(b.E or 0.0)
+                bds[bi].nu = float(b.nu or 0.0)
+                bds[bi].has_boundary = int(b.boundary)
+                self.body_index.append((si, bi))
+            cds = (ContactDesc * max(1, len(w.contacts)))()
+            for ci, c in enumerate(w.contacts):
+                cds[ci].body1, cds[ci].body2, cds[ci].mu = c.body1, c.body2, c.mu
+            keep += [bds, cds]
+            sims[si].grid = _grid_desc(w.dim, w.L, w.N, w.cpdi_spec, w.contact_spec, w.gravity)
+            sims[si].nbody = len(w.bodies)
+            sims[si].bodies = bds
+            sims[si].ncontact = len(w.contacts)
+            sims[si].contacts = cds
+        self._keep = keep
+        self._sims = sims
+        h = C.c_void_p()
+        _check(self.lib.mpm_gpu_create_batch(sims, len(self.workloads), precision, C.byref(h)))
+        self.h = h
+        _check(self.lib.mpm_gpu_set_sort_interval(self.h, sort_interval))
+        for gb, (si, bi) in enumerate(self.body_index):
+            self.upload(gb, self.workloads[si].bodies[bi])
+
+    # -- state transfer -------------------------------------------------------------
+    def _body(self, gb):
+        si, bi = self.body_index[gb]
+        return self.workloads[si].bodies[bi]
+
+    def upload(self, gb, body, T=None):
+        dim = self.dim
+        n = body.n
+        x = np.ascontiguousarray(np.asarray(body.x, np.float64)[:, :dim].T)
+        v = np.ascontiguousarray(np.asarray(body.xt, np.float64)[:, :dim].T)
+        V = np.ascontiguousarray(body.v, np.float64)
+        m = np.ascontiguousarray(body.m, np.float64)
+        ext = np.ascontiguousarray(extent_of(V, dim))
+        act = np.ascontiguousarray(body.active, np.int32)
+        Tp = None
+        if T is not None:
+            Tp = np.ascontiguousarray(np.asarray(T, np.float64).reshape(6, n))
+        _check(self.lib.mpm_gpu_upload(self.h, gb, x.ctypes.data_as(_dp), v.ctypes.data_as(_dp),
+                                       Tp.ctypes.data_as(_dp) if Tp is not None else None,
+                                       V.ctypes.data_as(_dp), m.ctypes.data_as(_dp), ext.ctypes.data_as(_dp),
+                                       act.ctypes.data_as(C.POINTER(C.c_int32))))
+
+    def step(self, dt, nsteps=1):
+        _check(self.lib.mpm_gpu_step(self.h, float(dt), int(nsteps)))
+
+    def points(self, gb):
+        """State of global body gb in original order: x, xt (n, dim); T (n, 6); v (n,); active."""
+        n = self._body(gb).n
+        dim = self.dim
+        x = np.zeros((dim, n))
+        v = np.zeros((dim, n))
+        T = np.zeros((6, n))
+        V = np.zeros(n)
+        a = np.zeros(n, np.int32)
+        _check(self.lib.mpm_gpu_download(self.h, gb, x.ctypes.data_as(_dp), v.ctypes.data_as(_dp),
+                                         T.ctypes.data_as(_dp), V.ctypes.data_as(_dp),
+                                         a.ctypes.data_as(C.POINTER(C.c_int32))))
+        return {"x": x.T.copy(), "xt": v.T.copy(), "T6": T.T.copy(), "v": V, "active": a}
+
+    def aux(self, gb):
+        n = self._body(gb).n
+        u = np.zeros((self.dim, n))
+        L = np.zeros((9, n))
+        _check(self.lib.mpm_gpu_download_aux(self.h, gb, u.ctypes.data_as(_dp), L.ctypes.data_as(_dp)))
+        return {"u": u.T.copy(), "L": L.T.copy()}
+
+    def connectivity(self, gb):
+        n = self._body(gb).n
+        nc = 2 ** self.dim
+        elem = np.zeros(n * nc, np.int32)
+        code = np.zeros(n, np.uint8)
+        _check(self.lib.mpm_gpu_connectivity(self.h, gb, elem.ctypes.data_as(C.POINTER(C.c_int32)),
+                                             code.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return elem.reshape(n, nc), code
+
+    # -- instrumentation --------------------------------------------------------------
+    def set_timing(self, on=True):
+        _check(self.lib.mpm_gpu_set_timing(self.h, int(on)))
+
+    def timing(self, cls):
+        n = C.c_int64()
+        ms = C.c_double()
+        _check(self.lib.mpm_gpu_get_timing(self.h, cls, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def stats(self):
+        an = C.c_int64()
+        ap = C.c_int64()
+        _check(self.lib.mpm_gpu_stats(self.h, C.byref(an), C.byref(ap)))
+        return an.value, ap.value
+
+    def synchronize(self):
+        _check(self.lib.mpm_gpu_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mpm_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stress6_to9(T6):
+    """(n, 6) symmetric xx, yy, zz, xy, xz, yz -> (n, 9) row-major."""
+    T6 = np.asarray(T6)
+    xx, yy, zz, xy, xz, yz = (T6[:, i] for i in range(6))
+    return np.stack([xx, xy, xz, xy, yy, yz, xz, yz, zz], axis=1)

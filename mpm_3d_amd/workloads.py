@@ -1,0 +1,205 @@
+"""Synthetic inputs and ``.points`` files for the explicit MPM step.
+
+``lattice_block`` places ``k`` points per cell per dimension at offsets ``(j+1/2)/k * h``
+inside an axis-aligned box, with ``V = h^D / k^D`` and ``m = rho * V`` (the layout of
+mpm_pygen/Grid3d.py:20-54). ``read_points``/``write_points`` handle the reference's
+``.points`` text format (default_points.cpp:104-193): a count line, then per point
+``m v x.. x_t.. active`` separated by single spaces.
+
+Named workloads (sizes from the benchmark configurations):
+
+* ``block_drop_2d``     -- 20x20 grid on 1x1 m, 0.4 m block at 2x2 points/cell, 0.3 m up.
+* ``two_block_3d``      -- 256^3 grid on a 1 m cube, two 0.3 m cubes at 2^3 points/cell,
+                           centres 0.4 m apart in x, velocities +-1 m/s (about 7.2M points).
+* ``column_3d``         -- 256x256x128 grid on 2x2x1 m, a column at 2^3 points/cell with
+                           positions jittered by +-0.25 cell from a seed; ``wide=True`` gives
+                           the 1.0x1.0x0.953125 m column of 15,990,784 points.
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class Body:
+    name: str
+    m: np.ndarray          # (n,)
+    v: np.ndarray          # (n,) volume
+    x: np.ndarray          # (n, dim)
+    xt: np.ndarray         # (n, dim) velocity
+    active: np.ndarray     # (n,) int32
+    E: Optional[float] = 1e5
+    nu: Optional[float] = 0.3
+    boundary: bool = True
+
+    @property
+    def n(self) -> int:
+        return int(self.m.shape[0])
+
+
+@dataclasses.dataclass
+class Contact:
+    body1: int
+    body2: int
+    mu: float
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    dim: int
+    L: Sequence[float]
+    N: Sequence[int]
+    bodies: List[Body]
+    contacts: List[Contact]
+    dt: float
+    steps: int
+    gravity: Sequence[float]
+    cpdi_spec: int = 1
+    contact_spec: int = 1
+
+    @property
+    def n_points(self) -> int:
+        return sum(b.n for b in self.bodies)
+
+
+def lattice_block(dim, h, lo, hi, k, rho, velocity=None, jitter=0.0, seed=0):
+    """Points at ``k`` per cell per dim on the lattice of cells of size ``h`` inside [lo, hi).
+
+    ``jitter`` moves every coordinate by U(-jitter, +jitter) * h (seeded)."""
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    h = np.asarray(h, np.float64)
+    axes = []
+    for d in range(dim):
+        c0 = int(np.floor(lo[d] / h[d] + 1e-9))
+        c1 = int(np.ceil(hi[d] / h[d] - 1e-9))
+        cells = np.arange(c0, c1)
+        offs = (np.arange(k) + 0.5) / k
+        coords = ((cells[:, None] + offs[None, :]) * h[d]).reshape(-1)
+        coords = coords[(coords >= lo[d]) & (coords < hi[d])]
+        axes.append(coords)
+    grids = np.meshgrid(*axes, indexing="ij")
+    # dim 0 fastest in the file order, as the reference's generators write them
+    x = np.stack([g.transpose(*reversed(range(dim))).reshape(-1) for g in grids], axis=1)
+    n = x.shape[0]
+    vol = float(np.prod(h)) / k ** dim
+    if jitter:
+        rng = np.random.default_rng(seed)
+        x = x + rng.uniform(-jitter, jitter, size=x.shape) * h[None, :dim]
+    m = np.full(n, rho * vol)
+    v = np.full(n, vol)
+    xt = np.zeros((n, dim))
+    if velocity is not None:
+        xt[:] = np.asarray(velocity, np.float64)[None, :dim]
+    return m, v, x, xt
+
+
+def block_drop_2d(points_per_cell=2, steps=2000):
+    L, N = (1.0, 1.0), (20, 20)
+    h = np.array([L[0] / N[0], L[1] / N[1]])
+    m, v, x, xt = lattice_block(2, h, (0.3, 0.3), (0.7, 0.7), points_per_cell, 1000.0)
+    body = Body("block", m, v, x, xt, np.ones(len(m), np.int32), 1e5, 0.3)
+    return Workload("block_drop_2d_20x20", 2, L, N, [body], [], 1e-4, steps, (0.0, -9.81))
+
+
+def two_block_3d(n_cells=256, gap_cells=None, steps=1000, E=1e6, mu=0.4, gravity=(0, 0, -9.81)):
+    """Two 0.3 m cubes, centres 0.4 m apart along x (or ``gap_cells`` cells apart at the
+    faces when given), velocities +-1 m/s, ContactHuang between them."""
+    L = (1.0, 1.0, 1.0)
+    N = (n_cells,) * 3
+    h = np.array([1.0 / n_cells] * 3)
+    half = 0.15
+    if gap_cells is None:
+        c1, c2 = 0.3, 0.7
+    else:
+        gap = gap_cells * h[0]
+        c1 = 0.5 - gap / 2 - half
+        c2 = 0.5 + gap / 2 + half
+    bodies = []
+    for name, cx, vx in (("block", c1, 1.0), ("block2", c2, -1.0)):
+        lo = (cx - half, 0.5 - half, 0.5 - half)
+        hi = (cx + half, 0.5 + half, 0.5 + half)
+        m, v, x, xt = lattice_block(3, h, lo, hi, 2, 1000.0, velocity=(vx, 0, 0))
+        bodies.append(Body(name, m, v, x, xt, np.ones(len(m), np.int32), E, 0.3))
+    return Workload(f"two_block_3d_{n_cells}^3", 3, L, N, bodies, [Contact(0, 1, mu)], 2e-5, steps,
+                    gravity)
+
+
+def column_3d(wide=True, steps=1000, seed=0xC0FFEE + 2, grid=(256, 256, 128), domain=(2.0, 2.0, 1.0),
+              size=None):
+    """Column collapse at h = 7.8125 mm: 2^3 points per cell jittered by +-0.25 cell (seeded),
+    E = 1e7 Pa, nu = 0.3, rho = 1500, dt = 1e-5, gravity -9.81 in z, CartesianBox walls.
+
+    ``wide`` selects the 1.0x1.0x0.953125 m column (15,990,784 points on the 256x256x128
+    grid over 2x2x1 m); otherwise the 0.5x0.5x0.9 m column (about 3.77M points).
+    ``size`` overrides the column dimensions (m); the column stands centred on the floor."""
+    N = tuple(grid)
+    L = tuple(domain)
+    h = np.array([L[i] / N[i] for i in range(3)])
+    if size is None:
+        size = (1.0, 1.0, 0.953125) if wide else (0.5, 0.5, 0.9)
+    wx, wy, hz = size
+    lo = (L[0] / 2 - wx / 2, L[1] / 2 - wy / 2, 0.0)
+    hi = (L[0] / 2 + wx / 2, L[1] / 2 + wy / 2, hz)
+    m, v, x, xt = lattice_block(3, h, lo, hi, 2, 1500.0, jitter=0.25, seed=seed)
+    body = Body("column", m, v, x, xt, np.ones(len(m), np.int32), 1e7, 0.3)
+    name = f"column_3d_{len(m)}pts_{N[0]}x{N[1]}x{N[2]}"
+    return Workload(name, 3, L, N, [body], [], 1e-5, steps, (0.0, 0.0, -9.81))
+
+
+def column_3d_sample(steps=4, seed=0xC0FFEE + 2):
+    """A 1/16-footprint sample of the 16M column: same h, material and dt; a
+    0.25x0.25x0.953125 m column on a 64x64x128 grid over 0.5x0.5x1 m (999,424 points)."""
+    return column_3d(steps=steps, seed=seed, grid=(64, 64, 128), domain=(0.5, 0.5, 1.0),
+                     size=(0.25, 0.25, 0.953125))
+
+
+def read_points(path, job_dim, grid_dim=None):
+    """Read a ``.points`` file like DefaultPoints::readFromFile (default_points.cpp:104-193).
+
+    Columns: ``m v x[DIM] x_t[DIM] active`` split on single spaces; a line with
+    ``GRID_DIM`` columns per vector is also accepted. Returns (m, v, x, xt, active) with
+    ``x``/``xt`` of width ``job_dim``."""
+    grid_dim = job_dim if grid_dim is None else grid_dim
+    with open(path) as f:
+        n = int(f.readline())
+        m = np.zeros(n)
+        v = np.zeros(n)
+        x = np.zeros((n, job_dim))
+        xt = np.zeros((n, job_dim))
+        act = np.zeros(n, np.int32)
+        for i in range(n):
+            line = f.readline().rstrip("\n")
+            tok = line.split(" ")
+            if len(tok) == 3 + 2 * job_dim:
+                d = job_dim
+            elif len(tok) == 3 + 2 * grid_dim:
+                d = grid_dim
+            else:
+                raise ValueError(f"unable to read line {i + 2} of {path}")
+            m[i] = float(tok[0])
+            v[i] = float(tok[1])
+            x[i, :d] = [float(t) for t in tok[2:2 + d]]
+            xt[i, :d] = [float(t) for t in tok[2 + d:2 + 2 * d]]
+            act[i] = int(float(tok[2 + 2 * d]))
+    return m, v, x, xt, act
+
+
+def write_points(path, m, v, x, xt, active):
+    """Write a ``.points`` file (17 significant digits, single-space separated)."""
+    n = len(m)
+    with open(path, "w") as f:
+        f.write(f"{n}\n")
+        for i in range(n):
+            cols = [m[i], v[i], *x[i], *xt[i]]
+            f.write(" ".join(f"{c:.17g}" for c in cols) + f" {int(active[i])}\n")
+
+
+def example_dir():
+    return os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests",
+                        "golden", "examples")

@@ -1,0 +1,141 @@
+"""GPU parity against the CPU oracle, through the C-ABI (libmpm_gpu.so).
+
+* connectivity: the element of every GIMP corner and the CPDI flag are bit-identical to
+  the oracle's (default_points.cpp:196-330, cartesian.cpp:219-253), on the shipped
+  examples, on jittered lattices and on hand-placed edge cases (points on node
+  coordinates, corners exactly at x = L, points outside the domain, near-wall points
+  that fall back to CPDI off).
+* steps: positions, velocities, stresses and volumes after N steps agree with the
+  oracle within the tolerances stated in tests/_harness.py.
+"""
+import numpy as np
+import pytest
+
+from mpm_3d_amd import workloads as W
+from tests._harness import compare_states, example_workload, gpu_of, oracle_of
+
+pytestmark = pytest.mark.gpu
+
+
+def _conn_check(w):
+    ora = oracle_of(w)
+    gpu = gpu_of(w)
+    for bi in range(len(w.bodies)):
+        eo, co = ora.connectivity(bi)
+        eg, cg = gpu.connectivity(bi)
+        np.testing.assert_array_equal(cg, co, err_msg=f"{w.name} body {bi}: CPDI codes")
+        np.testing.assert_array_equal(eg, eo, err_msg=f"{w.name} body {bi}: corner elements")
+    return ora, gpu
+
+
+@pytest.mark.parametrize("name", ["blockdrop2D", "collision2D", "blockdrop3D", "collision3D"])
+def test_connectivity_examples(name):
+    _conn_check(example_workload(name))
+
+
+def _edge_points(dim, L, N, rng, n_random=4000):
+    h = np.array([L[d] / N[d] for d in range(dim)])
+    pts = []
+    # random interior and near-wall points
+    pts.append(rng.uniform(0, 1, size=(n_random, dim)) * np.array(L[:dim]))
+    # points within one extent of every wall (CPDI off)
+    near = rng.uniform(0, 1, size=(400, dim)) * np.array(L[:dim])
+    near[:200, 0] = rng.uniform(0, 0.05 * h[0], 200)
+    near[200:, dim - 1] = L[dim - 1] - rng.uniform(0, 0.05 * h[dim - 1], 200)
+    pts.append(near)
+    # points on node coordinates and cell centres
+    k = rng.integers(1, N[0], size=(200, dim))
+    pts.append(k * h)
+    pts.append((k + 0.5) * h)
+    x = np.concatenate(pts)
+    return x
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_connectivity_edge_cases(dim):
+    rng = np.random.default_rng(1234 + dim)
+    L = (1.0, 0.75, 0.5)[:dim]
+    N = (20, 15, 10)[:dim]
+    x = _edge_points(dim, L, N, rng)
+    n = len(x)
+    h = np.array([L[d] / N[d] for d in range(dim)])
+    # volumes giving extents of 1/4 cell, some larger (up to 1.2 cells)
+    vol = np.full(n, float(np.prod(h)) / 2 ** dim)
+    vol[::7] *= 20.0
+    # corners exactly at x = L: place points at L - extent/2
+    ext = 0.5 * (np.sqrt(vol) if dim == 2 else np.cbrt(vol))
+    x[-50:, 0] = L[0] - 0.5 * ext[-50:]
+    # points outside the domain and exactly at L
+    x[-60:-55, 0] = L[0] + 0.01
+    x[-55:-50, 0] = L[0]
+    act = np.ones(n, np.int32)
+    act[::11] = 0
+    body = W.Body("pts", np.full(n, 1.0), vol, x, np.zeros((n, dim)), act, 1e5, 0.3, True)
+    g = (0.0, -9.81) if dim == 2 else (0.0, 0.0, -9.81)
+    w = W.Workload("edges", dim, L, N, [body], [], 1e-4, 1, g)
+    _conn_check(w)
+    # CPDI off everywhere
+    w.cpdi_spec = 0
+    _conn_check(w)
+
+
+@pytest.mark.parametrize("name,steps", [("blockdrop2D", 1), ("blockdrop2D", 200), ("collision2D", 200),
+                                        ("minimal2D", 100), ("blockdrop3D", 50), ("collision3D", 50)])
+def test_example_steps(name, steps):
+    w = example_workload(name)
+    ora = oracle_of(w)
+    gpu = gpu_of(w)
+    ora.step(w.dt, steps)
+    gpu.step(w.dt, steps)
+    compare_states(w, ora, gpu, steps * w.dt, label=f"{name}x{steps}")
+
+
+def test_jittered_block_3d_with_stress():
+    """A jittered 3D block with initial velocity and stress, walls and gravity."""
+    rng = np.random.default_rng(7)
+    h = np.array([1 / 24] * 3)
+    m, v, x, xt = W.lattice_block(3, h, (0.2, 0.25, 0.02), (0.6, 0.55, 0.4), 2, 1000.0, jitter=0.2, seed=3)
+    xt = rng.normal(0, 0.5, size=xt.shape)
+    body = W.Body("blk", m, v, x, xt, np.ones(len(m), np.int32), 1e6, 0.3, True)
+    w = W.Workload("jit3d", 3, (1.0, 1.0, 1.0), (24, 24, 24), [body], [], 2e-5, 40, (0.0, 0.0, -9.81))
+    ora = oracle_of(w)
+    gpu = gpu_of(w)
+    T9 = rng.normal(0, 1e3, size=(len(m), 3, 3))
+    T9 = 0.5 * (T9 + T9.transpose(0, 2, 1))
+    ora.set_points(0, T=T9.reshape(-1, 9))
+    T6 = np.stack([T9[:, 0, 0], T9[:, 1, 1], T9[:, 2, 2], T9[:, 0, 1], T9[:, 0, 2], T9[:, 1, 2]])
+    gpu.upload(0, body, T=T6)
+    ora.step(w.dt, 40)
+    gpu.step(w.dt, 40)
+    compare_states(w, ora, gpu, 40 * w.dt, label="jit3d")
+    # L of the last step
+    Lg = gpu.aux(0)["L"]
+    Lo = ora.points(0)["L"]
+    scale = np.max(np.abs(Lo)) + 1e-300
+    assert np.max(np.abs(Lg - Lo)) <= 1e-8 * scale
+
+
+def test_two_block_contact_engaged_3d():
+    """Two jittered cubes two cells apart closing at 2 m/s: ContactHuang engages."""
+    w = W.two_block_3d(n_cells=32, gap_cells=1, steps=60, E=1e6)
+    for b in w.bodies:
+        b.x = b.x + np.random.default_rng(5).uniform(-0.1, 0.1, b.x.shape) / 32
+    ora = oracle_of(w)
+    gpu = gpu_of(w)
+    ora.step(w.dt * 10, 60)
+    gpu.step(w.dt * 10, 60)
+    _, f = ora.contact(0)
+    assert np.abs(f).sum() > 0, "contact did not engage in the oracle"
+    compare_states(w, ora, gpu, 60 * w.dt * 10, label="contact3d")
+
+
+def test_sort_interval_invariance():
+    """The particle order on the device does not change results beyond rounding."""
+    w = example_workload("collision3D")
+    a = gpu_of(w, sort_interval=1)
+    b = gpu_of(w, sort_interval=0)
+    a.step(w.dt, 30)
+    b.step(w.dt, 30)
+    for bi in range(2):
+        pa, pb = a.points(bi), b.points(bi)
+        assert np.max(np.abs(pa["x"] - pb["x"])) < 1e-12
