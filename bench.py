@@ -28,13 +28,14 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-updates/s, 16M-pt 3D (FP32 vs FP64); batched solves/s + p99 latency"
 HBM_PEAK_GBS = 8000.0
 
-# algorithmic bytes (3D, FP64 form); see the per-kernel table in DESIGN.md
+# algorithmic bytes (3D, FP64 form) per launch, as (bytes per active point, bytes per
+# active node); the per-kernel table is in DESIGN.md
 BYTES = {
-    # kernel class: (bytes per active point, bytes per active node)
-    "p2g": (15 * 8, 7 * 8),     # read x3 v3 T6 V m extent; write m, p3, f3
-    "grid": (0, 13 * 8),        # read m, p3, f3; write v3, dv3
-    "g2p": (14 * 8 + 13 * 8, 6 * 8),  # read x3 v3 T6 V extent, write x3 v3 T6 V; read v3 dv3
-    "step": (224, 208),         # one fused particle pass + grid update per step
+    "pass": (224, 104),         # read x3 v3 T6 V m extent, write x3 v3 T6 V; read v3 dv3, write m p3 f3
+    "grid": (0, 104),           # read m p3 f3, write v3 dv3
+    "p2g": (120, 56),           # read x3 v3 T6 V m extent; write m p3 f3
+    "g2p": (216, 48),           # read x3 v3 T6 V extent, write x3 v3 T6 V; read v3 dv3
+    "step": (224, 208),         # one particle pass + one grid update
 }
 CLASSES = {0: "pass", 1: "grid", 2: "sort", 3: "p2g", 4: "g2p"}
 
@@ -68,6 +69,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--sort-interval", type=int, default=16)
+    ap.add_argument("--split", action="store_true", help="separate G2P and P2G passes instead of one fused pass")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the plain copy kernel (1 GiB, 10 launches) used to calibrate counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is "
@@ -75,11 +79,15 @@ def main():
         return 2
 
     from mpm_3d_amd import workloads as W
-    from mpm_3d_amd.gpu import GpuSim
+    from mpm_3d_amd.gpu import GpuSim, copy_probe
 
+    if args.copy_probe:
+        print(json.dumps({"copy_probe_gbps": copy_probe(1 << 30, 10), "bytes_per_launch": 2 * (1 << 30)}))
+        return 0
+    hbm_copy = copy_probe(1 << 30, 20)
     w = W.column_3d(wide=True)
     print(f"[bench] workload {w.name}: {w.n_points} points, grid {w.N}", file=sys.stderr, flush=True)
-    sim = GpuSim(w, sort_interval=args.sort_interval)
+    sim = GpuSim(w, sort_interval=args.sort_interval, fused=not args.split)
     if args.warmup:
         sim.step(w.dt, args.warmup)
     sim.synchronize()
@@ -127,9 +135,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": w.name, "points": int(w.n_points), "grid": list(w.N), "dt": w.dt,
-                   "form": "plain FP64", "sort_interval": args.sort_interval},
+                   "form": "plain FP64", "sort_interval": args.sort_interval,
+                   "particle_pass": "split" if args.split else "fused"},
         "roofline": roofline,
         "step_hbm_fraction": round(step_bytes / wall / 1e9 / HBM_PEAK_GBS, 4),
+        "hbm_copy_gbps": round(hbm_copy, 1),
         "kernels": kern,
     }
     sim.close()

@@ -102,8 +102,8 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
                    const double *T, const double *V, const double *m, const double *extent,
                    const int32_t *active);
 
-/* Advance every sim of the handle by nsteps steps of size dt (per-sim dt taken from
- * mpm_gpu_set_dt when set, else this dt). Replaces nsteps calls of ExplicitUSL::step
+/* Advance every sim of the handle by nsteps steps of size dt. Replaces nsteps calls of
+ * ExplicitUSL::step
  * (explicit_usl.cpp:48-87) with DefaultDriver gravity (default_driver.cpp:46-86). */
 int mpm_gpu_step(mpm_gpu_handle *h, double dt, int nsteps);
 
@@ -127,15 +127,36 @@ int mpm_gpu_connectivity(mpm_gpu_handle *h, int body, int32_t *corner_elem, uint
  * start of each mpm_gpu_step call). Affects speed only. */
 int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
 
+/* Execution options (speed only; results agree to rounding):
+ *   MPM_OPT_SORT_INTERVAL  as mpm_gpu_set_sort_interval
+ *   MPM_OPT_FUSED_PASS     1 (default): one particle pass per step doing G2P and the next
+ *                          step's P2G; 0: separate G2P and P2G passes */
+#define MPM_OPT_SORT_INTERVAL 1
+#define MPM_OPT_FUSED_PASS 2
+int mpm_gpu_set_option(mpm_gpu_handle *h, int key, int value);
+
 /* Kernel timing with HIP events on the handle's stream: enable, then read the number of
  * timed launches and their summed milliseconds per kernel class
- * (0 = particle pass G2P+P2G, 1 = grid update, 2 = sort, 3 = first P2G, 4 = last G2P). */
+ * (0 = fused particle pass G2P+P2G, 1 = grid update, 2 = sort, 3 = P2G pass, 4 = G2P pass). */
 int mpm_gpu_set_timing(mpm_gpu_handle *h, int enable);
 int mpm_gpu_get_timing(mpm_gpu_handle *h, int kernel_class, int64_t *launches, double *ms);
 
 /* Number of grid nodes that held mass in the last step (summed over bodies) and the
  * number of active points. */
 int mpm_gpu_stats(mpm_gpu_handle *h, int64_t *active_nodes, int64_t *active_points);
+
+/* Testing aid with no counterpart in the reference: evaluates, on the host, the same
+ * shape-function code the kernels run for one point (position x, GIMP half-width
+ * extent) and writes its node tuples -- node i,j,k (3 ints each), S, and gradS (3
+ * doubles each) -- up to max_tuples. Returns the tuple count, -100 when the point is
+ * outside the domain, or a negative MPM_ERR_* code. Needs no GPU. */
+int mpm_gpu_host_weights(const mpm_grid_desc *grid, const double *x, double extent, int32_t *node_ijk,
+                         double *S, double *G, int max_tuples);
+
+/* Measurement aid with no counterpart in the reference: times `reps` launches of a
+ * plain device copy of nbytes (read + write) and returns the achieved HBM GB/s. Its
+ * dispatches also calibrate the FETCH_SIZE / WRITE_SIZE counters. */
+int mpm_gpu_copy_probe(int64_t nbytes, int reps, double *gbps);
 
 int mpm_gpu_synchronize(mpm_gpu_handle *h);
 int mpm_gpu_destroy(mpm_gpu_handle *h);

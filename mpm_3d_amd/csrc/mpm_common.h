@@ -31,6 +31,8 @@ struct SimDev {
     double L[3];
     double h[3];
     double invh[3];
+    double NoverL[3];    // N / L, for the fast element search
+    double ftol[3];      // distance to an integer below which the exact search is used
     int NBn[3];          // node blocks per dimension
     int PNB[3];          // particle blocks per dimension
     long long nblk_off;  // first global node-block id of this sim
@@ -63,12 +65,18 @@ struct PBlk {  // a run of sorted particles sharing one particle block
 };
 
 template <int DIM> struct Cfg;
+// LDS patch layout: node (x, y, z) of the patch at x + P*y + ZS*z. In 3D the z-plane
+// stride is padded from 64 to 68 doubles: the 64 lanes of a wavefront sit in a 4x4x4 group
+// of dual cells, and with the padding their 8-byte accesses fall on each of the 32
+// double-wide bank slots exactly twice (x + 8y + 4z mod 32) instead of four times.
 template <> struct Cfg<3> {
     static constexpr int NB = 4;   // node-block edge (nodes)
     static constexpr int LNB = 2;  // log2(NB)
     static constexpr int PB = 4;   // particle-block edge (dual cells)
     static constexpr int P = 8;    // LDS patch edge (nodes): PB + 2 window + 2 drift
-    static constexpr int PATCH = 512;
+    static constexpr int ZS = 68;  // z-plane stride of the LDS patch
+    static constexpr int PATCH = 512;   // nodes in the patch
+    static constexpr int PSTR = 544;    // doubles per field plane of the patch (8 * ZS)
     static constexpr int NTS = 6;
 };
 template <> struct Cfg<2> {
@@ -76,7 +84,9 @@ template <> struct Cfg<2> {
     static constexpr int LNB = 3;
     static constexpr int PB = 8;
     static constexpr int P = 12;
+    static constexpr int ZS = 0;
     static constexpr int PATCH = 144;
+    static constexpr int PSTR = 144;
     static constexpr int NTS = 4;
 };
 
@@ -116,63 +126,83 @@ struct KArgs {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// sim-local node (i,j,k) -> offset inside the sim's node-block storage
+// sim-local node (i,j,k), -1 <= i <= N+1, -> offset inside the sim's node-block storage.
+// Storage is shifted by one node so the zero-weight slots of a 3-node window that hang
+// over the domain edge (nodes -1 and N+1) have a home; those padding nodes stay zero.
 template <int DIM>
-__device__ __forceinline__ long long node_addr(const SimDev &s, const int *ijk) {
+__host__ __device__ __forceinline__ long long node_addr(const SimDev &s, const int *ijk) {
     constexpr int NB = Cfg<DIM>::NB, L = Cfg<DIM>::LNB;
-    long long blk = ijk[0] >> L;
-    int loc = ijk[0] & (NB - 1);
+    const int i0 = ijk[0] + 1;
+    long long blk = i0 >> L;
+    int loc = i0 & (NB - 1);
     if (DIM >= 2) {
-        blk += (long long)(ijk[1] >> L) * s.NBn[0];
-        loc += (ijk[1] & (NB - 1)) << L;
+        const int i1 = ijk[1] + 1;
+        blk += (long long)(i1 >> L) * s.NBn[0];
+        loc += (i1 & (NB - 1)) << L;
     }
     if (DIM == 3) {
-        blk += (long long)(ijk[2] >> L) * s.NBn[0] * s.NBn[1];
-        loc += (ijk[2] & (NB - 1)) << (2 * L);
+        const int i2 = ijk[DIM - 1] + 1;
+        blk += (long long)(i2 >> L) * s.NBn[0] * s.NBn[1];
+        loc += (i2 & (NB - 1)) << (2 * L);
     }
     return blk * 64 + loc;
 }
 
 // CartesianLinear::inDomain (cartesian.cpp:245-253): 0 <= x <= L in every dimension
 template <int DIM>
-__device__ __forceinline__ bool in_domain(const SimDev &s, const double *x) {
+__host__ __device__ __forceinline__ bool in_domain(const SimDev &s, const double *x) {
     bool in = true;
 #pragma unroll
     for (int d = 0; d < DIM; d++) in = in && (x[d] <= s.L[d] && x[d] >= 0.0);
     return in;
 }
 
+// floor((x / L) * N) exactly as the reference rounds it (cartesian.cpp:226-236), for
+// 0 <= x <= L. q = x * (N/L) differs from fl(fl(x/L) * N) by at most 4 ulp(N); when q is
+// farther than ftol = N * 2^-48 from an integer both floors agree, otherwise the
+// reference's division is evaluated.
+__host__ __device__ __forceinline__ int elem_floor(const SimDev &s, int d, double x) {
+    const double q = x * s.NoverL[d];
+    const double f = floor(q);
+    const double r = q - f;
+    if (r > s.ftol[d] && r < 1.0 - s.ftol[d]) return (int)f;
+    return (int)floor(x / s.L[d] * (double)s.N[d]);
+}
+
 // CartesianLinear::cartesianWhichElement (cartesian.cpp:219-240), including the
 // double-precision accumulation of the element id. -1 unless 0 <= x < L.
 template <int DIM>
-__device__ __forceinline__ long long which_element(const SimDev &s, const double *x) {
+__host__ __device__ __forceinline__ long long which_element(const SimDev &s, const double *x) {
 #pragma unroll
     for (int d = 0; d < DIM; d++)
         if (!(x[d] < s.L[d] && x[d] >= 0.0)) return -1;
-    long long e = (long long)floor(x[0] / s.L[0] * (double)s.N[0]);
-    if (DIM >= 2) e = (long long)((double)e + floor(x[1] / s.L[1] * (double)s.N[1]) * (double)s.N[0]);
+    long long e = elem_floor(s, 0, x[0]);
+    if (DIM >= 2) e = (long long)((double)e + (double)elem_floor(s, 1, x[1]) * (double)s.N[0]);
     if (DIM == 3)
-        e = (long long)((double)e + floor(x[2] / s.L[2] * (double)s.N[2]) * (double)s.N[1] * (double)s.N[0]);
+        e = (long long)((double)e + (double)elem_floor(s, DIM - 1, x[DIM - 1]) * (double)s.N[1] * (double)s.N[0]);
     return e;
 }
 
 // Separable GIMP/CPDI weights of one point (default_points.cpp:264-324 summed per node):
-// per dimension, nodes base..base+cnt-1 (cnt <= 3) carry
+// per dimension the window is the 3 nodes base, base+1, base+2 with
 //   sw = hat(x - a) + hat(x + a)      and      dw = hat(x + a) - hat(x - a)
-// with a = 0.5*extent; then S = prod sw / 2^D and G_d = dw_d prod_{e!=d} sw_e * gscale,
+// (a = 0.5*extent); S = prod sw / 2^D and G_d = dw_d prod_{e!=d} sw_e * gscale with
 // gscale = 1 / (2^D * 0.5 * extent). Each corner's hat values are those of the element
 // that CartesianLinear::whichElement assigns to the corner, so a corner exactly at x = L
-// contributes nothing, as in the reference.
+// contributes nothing, as in the reference. base is the point's dual cell
+// floor(x/h - 1/2), moved into [ep-1, em] so both corner elements lie in the window;
+// slots no corner reaches carry exact zeros. The fixed 3^D stencil keeps the node loops
+// free of per-lane branches. Points whose corners span more than 2 elements in some
+// dimension (extent > h) take the per-tuple path.
 template <int DIM> struct Win {
     int base[DIM];
-    int cnt[DIM];
     double sw[DIM][3];
     double dw[DIM][3];
 };
 
 // cpdi_out tells the tuple path whether the point uses its corners (CPDI on) or not.
 template <int DIM>
-__device__ __forceinline__ int point_window(const SimDev &s, const double *x, double ext, Win<DIM> &w,
+__host__ __device__ __forceinline__ int point_window(const SimDev &s, const double *x, double ext, Win<DIM> &w,
                                             bool &cpdi_out) {
     cpdi_out = false;
     if (!in_domain<DIM>(s, x)) return MODE_LEFT;
@@ -188,47 +218,48 @@ __device__ __forceinline__ int point_window(const SimDev &s, const double *x, do
     }
     if (!cin) return MODE_TUP;  // a corner leaves the domain: CPDI off (default_points.cpp:220-235)
     cpdi_out = true;
+    int mode = MODE_SEP;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
         const bool vm = xm[d] < s.L[d], vp = xp[d] < s.L[d];
-        if (!vm && !vp) return MODE_NONE;
-        const int em = vm ? (int)floor(xm[d] / s.L[d] * (double)s.N[d]) : 0;
-        const int ep = vp ? (int)floor(xp[d] / s.L[d] * (double)s.N[d]) : 0;
-        if ((vm && em >= s.N[d]) || (vp && ep >= s.N[d])) return MODE_TUP;
+        if (!vm && !vp) mode = MODE_NONE;
+        const int em = vm ? elem_floor(s, d, xm[d]) : 0;
+        const int ep = vp ? elem_floor(s, d, xp[d]) : 0;
+        if ((vm && em >= s.N[d]) || (vp && ep >= s.N[d])) mode = mode == MODE_NONE ? MODE_NONE : MODE_TUP;
         const int lo = vm ? em : ep;
         const int hi = vp ? ep : em;
-        const int cnt = hi - lo + 2;
-        if (cnt > 3) return MODE_TUP;
-        w.base[d] = lo;
-        w.cnt[d] = cnt;
+        if (hi - lo > 1) mode = mode == MODE_NONE ? MODE_NONE : MODE_TUP;
+        int b = (int)floor(x[d] * s.invh[d] - 0.5);
+        b = b < hi - 1 ? hi - 1 : (b > lo ? lo : b);
+        w.base[d] = b;
         double s0 = 0, s1 = 0, s2 = 0, d0 = 0, d1 = 0, d2 = 0;
         const double hd = s.h[d], ih = s.invh[d];
         if (vm) {
             const double h0 = 1.0 - fabs((xm[d] - hd * em) * ih);
             const double h1 = 1.0 - fabs((xm[d] - hd * (em + 1)) * ih);
-            const int k = em - lo;  // 0 or 1
-            s0 += k == 0 ? h0 : 0.0;
-            s1 += k == 0 ? h1 : h0;
-            s2 += k == 0 ? 0.0 : h1;
-            d0 -= k == 0 ? h0 : 0.0;
-            d1 -= k == 0 ? h1 : h0;
-            d2 -= k == 0 ? 0.0 : h1;
+            const bool k0 = em == b;  // slot of the corner's element: 0 or 1
+            s0 += k0 ? h0 : 0.0;
+            s1 += k0 ? h1 : h0;
+            s2 += k0 ? 0.0 : h1;
+            d0 -= k0 ? h0 : 0.0;
+            d1 -= k0 ? h1 : h0;
+            d2 -= k0 ? 0.0 : h1;
         }
         if (vp) {
             const double h0 = 1.0 - fabs((xp[d] - hd * ep) * ih);
             const double h1 = 1.0 - fabs((xp[d] - hd * (ep + 1)) * ih);
-            const int k = ep - lo;
-            s0 += k == 0 ? h0 : 0.0;
-            s1 += k == 0 ? h1 : h0;
-            s2 += k == 0 ? 0.0 : h1;
-            d0 += k == 0 ? h0 : 0.0;
-            d1 += k == 0 ? h1 : h0;
-            d2 += k == 0 ? 0.0 : h1;
+            const bool k0 = ep == b;
+            s0 += k0 ? h0 : 0.0;
+            s1 += k0 ? h1 : h0;
+            s2 += k0 ? 0.0 : h1;
+            d0 += k0 ? h0 : 0.0;
+            d1 += k0 ? h1 : h0;
+            d2 += k0 ? 0.0 : h1;
         }
         w.sw[d][0] = s0; w.sw[d][1] = s1; w.sw[d][2] = s2;
         w.dw[d][0] = d0; w.dw[d][1] = d1; w.dw[d][2] = d2;
     }
-    return MODE_SEP;
+    return mode;
 }
 
 // Per-tuple evaluation, exactly the reference's COO construction for one point
@@ -237,7 +268,7 @@ __device__ __forceinline__ int point_window(const SimDev &s, const double *x, do
 // fn(node_ijk, S, G[DIM]) is called per tuple. Used for CPDI-off points and the rare
 // shapes the separable form does not cover.
 template <int DIM, class Fn>
-__device__ void for_each_tuple(const SimDev &s, const double *x, double ext, bool cpdi, Fn &&fn) {
+__host__ __device__ void for_each_tuple(const SimDev &s, const double *x, double ext, bool cpdi, Fn &&fn) {
     constexpr int NC = 1 << DIM;
     const int ncorner = cpdi ? NC : 1;
     for (int c = 0; c < ncorner; c++) {

@@ -50,7 +50,8 @@ struct Layout {  // sizes derived from the descriptors, shared by create and foo
     std::vector<SimDev> sims;
     std::vector<BodyDev> bodies;
     std::vector<ContactDev> contacts;
-    long long np = 0, nt = 0, nblk_total = 0, nkeys = 0;
+    long long np = 0, nt = 0, nblk_total = 0, nkeys = 0, npblk_total = 0;
+    bool any_b1 = false;
 };
 
 int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
@@ -82,7 +83,9 @@ int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
                 s.L[d] = g.L[d];
                 s.h[d] = g.L[d] / g.N[d];
                 s.invh[d] = 1.0 / s.h[d];
-                s.NBn[d] = (int)ceil_div(g.N[d] + 1, NB);
+                s.NoverL[d] = (double)g.N[d] / g.L[d];
+                s.ftol[d] = std::ldexp((double)g.N[d], -48);
+                s.NBn[d] = (int)ceil_div(g.N[d] + 3, NB);  // nodes -1..N+1
                 s.PNB[d] = (int)ceil_div(g.N[d] + 1, PB);
                 nblk *= s.NBn[d];
                 pnb *= s.PNB[d];
@@ -111,6 +114,7 @@ int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
                 cd.body1 == cd.body2)
                 return fail(MPM_ERR_ARG, "sim %d contact %d: bodies must be two distinct sim-local indices", i, c);
             is_b1[cd.body1] = 1;
+            Lo.any_b1 = true;
             ContactDev C{cd.body1, cd.body2, cd.mu};
             Lo.contacts.push_back(C);
         }
@@ -145,7 +149,8 @@ int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
     Lo.np = p_off;
     Lo.nt = grid_base;
     Lo.nblk_total = nblk_off;
-    Lo.nkeys = pblk_off + 1;  // + inactive bucket
+    Lo.npblk_total = pblk_off;
+    Lo.nkeys = pblk_off * (1ll << Lo.dim) + 1;  // (block, octant) keys + inactive bucket
     if (Lo.np >= (1ll << 31) - 1) return fail(MPM_ERR_ARG, "at most 2^31-2 points per handle");
     if (Lo.nkeys >= (1ll << 31) - 1) return fail(MPM_ERR_ARG, "grid too large for one handle");
     return MPM_OK;
@@ -161,7 +166,8 @@ long long footprint_bytes(const Layout &Lo, int precision) {
     b += Lo.np * (4 + 4);                  // sort key + rank
     b += Lo.nt * (nacc + nout) * 8;        // grid
     b += Lo.nblk_total * (4 + 2 * 4);      // node-block flags + two lists
-    b += Lo.nkeys * (4 + 4 + sizeof(PBlk)); // histogram, offsets, block list
+    b += Lo.nkeys * (4 + 4);               // histogram, offsets
+    b += Lo.npblk_total * sizeof(PBlk);    // block list
     return b;
 }
 
@@ -198,6 +204,7 @@ struct mpm_gpu_handle {
     long long stage_elems = 0;
     int par = 0;
     int sort_interval = 16;
+    int fused = 1;  // 1: one particle pass per step (G2P then P2G); 0: two passes
     bool sims_dirty = true;
     std::vector<char> uploaded;
     std::vector<std::vector<double>> x0;  // per body, dim planes (for displacement)
@@ -314,7 +321,8 @@ int launch_sort(mpm_gpu_handle *h) {
         }
     }
     hipLaunchKernelGGL(k_make_plist, dim3((unsigned)ceil_div(nk, 256)), dim3(256), 0, h->stream, A, h->bcount,
-                       h->boff, h->plist, h->npblk, nk);
+                       h->boff, h->plist, h->npblk, nk, 1 << DIM);
+    hipLaunchKernelGGL(k_clear, dim3((unsigned)ceil_div(nk, 256)), dim3(256), 0, h->stream, h->bcount, nk);
     hipLaunchKernelGGL(k_bin_scatter<PF<DIM>::N>, dim3(gp), dim3(256), 0, h->stream, np, h->P, h->P2, h->act,
                        h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff);
     std::swap(h->P, h->P2);
@@ -326,40 +334,63 @@ int launch_sort(mpm_gpu_handle *h) {
 
 constexpr int PASS_GRID = 2048;  // persistent workgroups for the particle passes
 
+template <int DIM, bool G2P, bool P2G>
+void launch_particles(mpm_gpu_handle *h, int want_L) {
+    KArgs A = kargs(h);
+    if (h->lo.any_b1)
+        hipLaunchKernelGGL((k_particles<DIM, G2P, P2G, true>), dim3(PASS_GRID), dim3(256), 0, h->stream, A, want_L);
+    else
+        hipLaunchKernelGGL((k_particles<DIM, G2P, P2G, false>), dim3(PASS_GRID), dim3(256), 0, h->stream, A, want_L);
+}
+
+// Step sequence with the list parity `par`: a P2G half marks node blocks into list[par];
+// k_grid(par) consumes it and clears list[par^1] for the next P2G.
+//   sort; P2G(par); { grid(par); par^=1; [sort]; G2P+P2G(par) } ...; grid(par); par^=1; G2P
 template <int DIM>
 int launch_step(mpm_gpu_handle *h, int nsteps) {
+    Timer t{};
+    int rc = launch_sort<DIM>(h);
+    if (rc) return rc;
+    rc = timer_begin(h, KC_P2G, t);
+    if (rc) return rc;
+    launch_particles<DIM, false, true>(h, 0);
+    if ((rc = timer_end(h, t))) return rc;
     for (int k = 0; k < nsteps; k++) {
-        const bool first = k == 0;
         const bool last = k == nsteps - 1;
-        int rc;
-        if (first || (h->sort_interval > 0 && k % h->sort_interval == 0)) {
-            rc = launch_sort<DIM>(h);
-            if (rc) return rc;
+        if ((rc = timer_begin(h, KC_GRID, t))) return rc;
+        {
+            KArgs Ag = kargs(h);
+            if (last) {
+                hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(64), 0, h->stream, h->stats, 4);
+            } else {
+                Ag.stats = nullptr;
+            }
+            hipLaunchKernelGGL(k_grid<DIM>, dim3(PASS_GRID), dim3(256), 0, h->stream, Ag);
         }
-        Timer t{};
-        KArgs A = kargs(h);
-        rc = timer_begin(h, KC_P2G, t);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_p2g<DIM>, dim3(PASS_GRID), dim3(256), 0, h->stream, A);
-        rc = timer_end(h, t);
-        if (rc) return rc;
-        if (last) {
-            HIP_TRY(hipMemsetAsync(h->stats, 0, sizeof(unsigned long long), h->stream));
-        }
-        rc = timer_begin(h, KC_GRID, t);
-        if (rc) return rc;
-        KArgs Ag = kargs(h);
-        if (!last) Ag.stats = nullptr;
-        hipLaunchKernelGGL(k_grid<DIM>, dim3(PASS_GRID), dim3(256), 0, h->stream, Ag);
-        rc = timer_end(h, t);
-        if (rc) return rc;
-        rc = timer_begin(h, KC_G2P, t);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_g2p<DIM>, dim3(PASS_GRID), dim3(256), 0, h->stream, A, last ? 1 : 0);
-        rc = timer_end(h, t);
-        if (rc) return rc;
-        HIP_TRY(hipGetLastError());
+        if ((rc = timer_end(h, t))) return rc;
         h->par ^= 1;
+        if (last) {
+            if ((rc = timer_begin(h, KC_G2P, t))) return rc;
+            launch_particles<DIM, true, false>(h, 1);
+            if ((rc = timer_end(h, t))) return rc;
+        } else {
+            if (h->sort_interval > 0 && (k + 1) % h->sort_interval == 0) {
+                if ((rc = launch_sort<DIM>(h))) return rc;
+            }
+            if (h->fused) {
+                if ((rc = timer_begin(h, KC_PASS, t))) return rc;
+                launch_particles<DIM, true, true>(h, 0);
+                if ((rc = timer_end(h, t))) return rc;
+            } else {
+                if ((rc = timer_begin(h, KC_G2P, t))) return rc;
+                launch_particles<DIM, true, false>(h, 0);
+                if ((rc = timer_end(h, t))) return rc;
+                if ((rc = timer_begin(h, KC_P2G, t))) return rc;
+                launch_particles<DIM, false, true>(h, 0);
+                if ((rc = timer_end(h, t))) return rc;
+            }
+        }
+        HIP_TRY(hipGetLastError());
     }
     return MPM_OK;
 }
@@ -404,7 +435,7 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     TRY(dalloc(&h->ncount, 2));
     TRY(dalloc(&h->bcount, L.nkeys));
     TRY(dalloc(&h->boff, L.nkeys));
-    TRY(dalloc(&h->plist, L.nkeys));
+    TRY(dalloc(&h->plist, L.npblk_total));
     TRY(dalloc(&h->npblk, 1));
     TRY(dalloc(&h->stats, 4));
     for (long long n = L.nkeys; n > 4096;) {
@@ -559,6 +590,21 @@ int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval) {
     if (interval < 0) return fail(MPM_ERR_ARG, "interval must be >= 0");
     h->sort_interval = interval;
     return MPM_OK;
+}
+
+int mpm_gpu_set_option(mpm_gpu_handle *h, int key, int value) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    switch (key) {
+        case MPM_OPT_SORT_INTERVAL:
+            if (value < 0) return fail(MPM_ERR_ARG, "sort interval must be >= 0");
+            h->sort_interval = value;
+            return MPM_OK;
+        case MPM_OPT_FUSED_PASS:
+            h->fused = value != 0;
+            return MPM_OK;
+        default:
+            return fail(MPM_ERR_ARG, "unknown option %d", key);
+    }
 }
 
 int mpm_gpu_step(mpm_gpu_handle *h, double dt, int nsteps) {
@@ -737,6 +783,100 @@ int mpm_gpu_stats(mpm_gpu_handle *h, int64_t *active_nodes, int64_t *active_poin
         *active_points = (int64_t)c;
     }
     return MPM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+__global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                    long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" {
+
+int mpm_gpu_copy_probe(int64_t nbytes, int reps, double *gbps) {
+    if (nbytes < 16 || reps < 1 || !gbps) return fail(MPM_ERR_ARG, "need nbytes >= 16, reps >= 1, gbps");
+    const long long n = nbytes / 16;
+    double2 *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipMalloc(&a, n * 16));
+    HIP_TRY(hipMalloc(&b, n * 16));
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipMemsetAsync(a, 1, n * 16, st));
+    const int grid = 256 * 16;
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, st, a, b, n);  // warm-up
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, st, a, b, n);
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    *gbps = 2.0 * n * 16 * reps / (ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return MPM_OK;
+}
+
+int mpm_gpu_host_weights(const mpm_grid_desc *grid, const double *x, double extent, int32_t *node_ijk, double *S,
+                         double *G, int max_tuples) {
+    if (!grid || !x || !node_ijk || !S || !G) return fail(MPM_ERR_ARG, "null argument");
+    mpm_body_desc bd{1, 0, 0.0, 0.0, 0};
+    mpm_sim_desc sd{};
+    sd.grid = *grid;
+    sd.nbody = 1;
+    sd.bodies = &bd;
+    Layout lo;
+    int rc = build_layout(&sd, 1, lo);
+    if (rc) return rc;
+    const SimDev &s = lo.sims[0];
+    int count = 0;
+    auto emit = [&](const int *nijk, double Sv, const double *Gv, int dim) {
+        if (count < max_tuples) {
+            for (int d = 0; d < 3; d++) {
+                node_ijk[3 * count + d] = d < dim ? nijk[d] : 0;
+                G[3 * count + d] = d < dim ? Gv[d] : 0.0;
+            }
+            S[count] = Sv;
+        }
+        count++;
+    };
+    if (s.dim == 3) {
+        Win<3> w;
+        bool cpdi;
+        const int mode = point_window<3>(s, x, extent, w, cpdi);
+        if (mode == MODE_LEFT) return -100;
+        if (mode == MODE_SEP) {
+            for_each_sep<3>(w, extent, [&](const int *off, double Sv, const double *Gv) {
+                int n[3] = {w.base[0] + off[0], w.base[1] + off[1], w.base[2] + off[2]};
+                emit(n, Sv, Gv, 3);
+            });
+        } else if (mode == MODE_TUP) {
+            for_each_tuple<3>(s, x, extent, cpdi, [&](const int *n, double Sv, const double *Gv) { emit(n, Sv, Gv, 3); });
+        }
+    } else {
+        Win<2> w;
+        bool cpdi;
+        const int mode = point_window<2>(s, x, extent, w, cpdi);
+        if (mode == MODE_LEFT) return -100;
+        if (mode == MODE_SEP) {
+            for_each_sep<2>(w, extent, [&](const int *off, double Sv, const double *Gv) {
+                int n[2] = {w.base[0] + off[0], w.base[1] + off[1]};
+                emit(n, Sv, Gv, 2);
+            });
+        } else if (mode == MODE_TUP) {
+            for_each_tuple<2>(s, x, extent, cpdi, [&](const int *n, double Sv, const double *Gv) { emit(n, Sv, Gv, 2); });
+        }
+    }
+    return count;
 }
 
 int mpm_gpu_synchronize(mpm_gpu_handle *h) {

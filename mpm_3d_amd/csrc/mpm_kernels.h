@@ -1,15 +1,22 @@
 // mpm_kernels.h -- the explicit USL step as HIP kernels for gfx950 (FP64 form).
 //
-// One step (explicit_usl.cpp:48-87) is three launches over resident HBM state:
-//   k_p2g   map (default_points.cpp:196-330) + P2G (explicit_usl.cpp:112-190), and the
-//           contact-normal scatter gradS*m (contact_huang.cpp:80-99): one workgroup per
-//           particle block accumulates into an LDS node patch with FP64 LDS atomics, then
-//           flushes the patch to the grid with FP64 global atomics and marks node blocks.
-//   k_grid  per touched node: ContactHuang (contact_huang.cpp:104-197), CartesianBox
-//           (cartesian_box.cpp:73-85), moveGrid (explicit_usl.cpp:233-264); one wavefront
-//           per 64-node block; consumes and re-zeroes the accumulators.
-//   k_g2p   movePoints, calculateStrainRate, updateDensity, gravity, updateStress
-//           (explicit_usl.cpp:266-355, isolin.cpp:50-78).
+// One step (explicit_usl.cpp:48-87) is two launches over resident HBM state:
+//   k_grid       per touched node: body force m g, ContactHuang (contact_huang.cpp:104-197),
+//                CartesianBox (cartesian_box.cpp:73-85), moveGrid (explicit_usl.cpp:233-264);
+//                one wavefront per 64-node block; consumes and re-zeroes the accumulators.
+//   k_particles  one workgroup per particle block (persistent over the block list):
+//                G2P from an LDS copy of the block's node patch -- movePoints,
+//                calculateStrainRate, updateDensity, gravity, updateStress
+//                (explicit_usl.cpp:266-355, isolin.cpp:50-78) -- then, for the next step,
+//                the map (default_points.cpp:196-330) and P2G (explicit_usl.cpp:112-190)
+//                with the contact-normal scatter gradS*m (contact_huang.cpp:80-99) into an
+//                LDS accumulator patch (FP64 LDS atomics), flushed to the grid with FP64
+//                global atomics. The first step of a call runs the P2G half alone, the
+//                last the G2P half alone.
+//
+// The nodal body force: every point carries b = g (default_driver.cpp:109-119), so
+// f_i = sum_p S_ip m_p g = m_i g; k_grid adds m_i g once per node instead of scattering
+// m_p g from every point.
 #pragma once
 
 #include "mpm_common.h"
@@ -30,8 +37,8 @@ __device__ void mark_range(const KArgs &A, const SimDev &s, const int *lo, const
     int b0[3] = {0, 0, 0}, b1[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        b0[d] = lo[d] >> L;
-        b1[d] = hi[d] >> L;
+        b0[d] = (lo[d] + 1) >> L;  // node storage is shifted by one (node_addr)
+        b1[d] = (hi[d] + 1) >> L;
     }
     for (int k = b0[2]; k <= b1[2]; k++)
         for (int j = b0[1]; j <= b1[1]; j++)
@@ -53,24 +60,78 @@ __device__ __forceinline__ void patch_origin(const SimDev &s, long long lkey, in
     }
 }
 
-// the per-point P2G payload: m, m v, m g, V T
+template <int DIM>
+__host__ __device__ __forceinline__ int patch_index(const int *nijk, const int *org) {
+    constexpr int P = Cfg<DIM>::P;
+    int li = nijk[0] - org[0] + P * (nijk[1] - org[1]);
+    if (DIM == 3) li += Cfg<DIM>::ZS * (nijk[DIM - 1] - org[DIM - 1]);
+    return li;
+}
+
+// one point's state (FP64 form)
+template <int DIM> struct PState {
+    double x[DIM], v[DIM], T6[6], V, m, ext;
+};
+
+template <int DIM>
+__device__ __forceinline__ void load_state(const KArgs &A, long long p, PState<DIM> &q) {
+    using F = PF<DIM>;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        q.x[d] = A.P[(F::X + d) * A.np + p];
+        q.v[d] = A.P[(F::V + d) * A.np + p];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) q.T6[c] = 0.0;
+#pragma unroll
+    for (int c = 0; c < F::NTS; c++) q.T6[c] = A.P[(F::T + c) * A.np + p];
+    q.V = A.P[F::VOL * A.np + p];
+    q.m = A.P[F::MASS * A.np + p];
+    q.ext = A.P[F::EXT * A.np + p];
+}
+
+template <int DIM>
+__device__ __forceinline__ void store_state(const KArgs &A, long long p, const PState<DIM> &q) {
+    using F = PF<DIM>;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        A.P[(F::X + d) * A.np + p] = q.x[d];
+        A.P[(F::V + d) * A.np + p] = q.v[d];
+    }
+#pragma unroll
+    for (int c = 0; c < F::NTS; c++) A.P[(F::T + c) * A.np + p] = q.T6[c];
+    A.P[F::VOL * A.np + p] = q.V;
+}
+
+// the per-point P2G payload: m, m v and V T (explicit_usl.cpp:132-161)
 template <int DIM> struct P2GVals {
     double m;
     double mv[DIM];
-    double mg[DIM];
     double VT[DIM][DIM];
 };
 
 template <int DIM>
-__device__ __forceinline__ void p2g_node_values(const P2GVals<DIM> &q, double S, const double *G, double *out,
-                                                 bool b1) {
-    // out: m, p[DIM], f[DIM], n[DIM]
+__device__ __forceinline__ void p2g_payload(const PState<DIM> &q, P2GVals<DIM> &pv) {
+    pv.m = q.m;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) pv.mv[d] = q.v[d] * q.m;
+    const double Tf[3][3] = {{q.T6[0], q.T6[3], q.T6[4]}, {q.T6[3], q.T6[1], q.T6[5]}, {q.T6[4], q.T6[5], q.T6[2]}};
+#pragma unroll
+    for (int r = 0; r < DIM; r++)
+#pragma unroll
+        for (int c = 0; c < DIM; c++) pv.VT[r][c] = Tf[r][c] * q.V;
+}
+
+// node values of one P2G tuple: m, p[DIM], f[DIM] (stress divergence only), n[DIM]
+template <int DIM>
+__host__ __device__ __forceinline__ void p2g_node_values(const P2GVals<DIM> &q, double S, const double *G,
+                                                          double *out, bool b1) {
     out[0] = S * q.m;
 #pragma unroll
     for (int d = 0; d < DIM; d++) out[1 + d] = S * q.mv[d];
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        double f = S * q.mg[d];
+        double f = 0.0;
 #pragma unroll
         for (int c = 0; c < DIM; c++) f = fma(-q.VT[d][c], G[c], f);
         out[1 + DIM + d] = f;
@@ -79,197 +140,415 @@ __device__ __forceinline__ void p2g_node_values(const P2GVals<DIM> &q, double S,
     for (int d = 0; d < DIM; d++) out[1 + 2 * DIM + d] = b1 ? G[d] * q.m : 0.0;
 }
 
+// Visit the 3^DIM window slot by slot with the loop kept rolled (weights picked by
+// selects); used on the rare global-memory path so its loads do not set the kernel's
+// register budget. fn(off, S, G) with off the slot offsets from w.base.
+__host__ __device__ __forceinline__ double pick3(const double *a, int i) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
+
+template <int DIM, class Fn>
+__host__ __device__ __forceinline__ void for_each_sep(const Win<DIM> &w, double ext, Fn &&fn) {
+    const double gscale = 1.0 / ((double)(1 << DIM) * 0.5 * ext);
+    const double sscale = 1.0 / (double)(1 << DIM);
+    constexpr int NS = DIM == 3 ? 27 : 9;
+#pragma unroll 1
+    for (int k = 0; k < NS; k++) {
+        const int c0 = k % 3, c1 = (k / 3) % 3, c2 = k / 9;
+        const double sx = pick3(w.sw[0], c0), dx = pick3(w.dw[0], c0);
+        const double sy = pick3(w.sw[1], c1), dy = pick3(w.dw[1], c1);
+        const double sz = DIM == 3 ? pick3(w.sw[DIM - 1], c2) : 1.0;
+        const double dz = DIM == 3 ? pick3(w.dw[DIM - 1], c2) : 0.0;
+        const double S = sx * (sy * sz) * sscale;
+        double G[DIM];
+        G[0] = dx * (sy * sz) * gscale;
+        G[1] = sx * (dy * sz) * gscale;
+        if (DIM == 3) G[DIM - 1] = sx * (sy * dz) * gscale;
+        int off[DIM];
+        off[0] = c0;
+        off[1] = c1;
+        if (DIM == 3) off[DIM - 1] = c2;
+        fn(off, S, G);
+    }
+}
+
+// Row coefficients of the separable window: along a row (fixed c1, c2) the x-slot a has
+//   S = sw0[a] * Su,  G0 = dw0[a] * gx,  G1 = sw0[a] * gy,  G2 = sw0[a] * gz
+template <int DIM> struct RowW {
+    double Su, gx, gy, gz;
+};
+
 template <int DIM>
-__global__ __launch_bounds__(256) void k_p2g(KArgs A) {
-    constexpr int P = Cfg<DIM>::P, PATCH = Cfg<DIM>::PATCH, NACC = GF<DIM>::NACC;
-    using F = PF<DIM>;
-    __shared__ double lacc[NACC * PATCH];
+__device__ __forceinline__ RowW<DIM> row_weights(const Win<DIM> &w, int c1, int c2, double sscale, double gscale) {
+    const double sz = DIM == 3 ? w.sw[DIM - 1][c2] : 1.0;
+    const double dz = DIM == 3 ? w.dw[DIM - 1][c2] : 0.0;
+    const double syz = w.sw[1][c1] * sz;
+    RowW<DIM> r;
+    r.Su = syz * sscale;
+    r.gx = syz * gscale;
+    r.gy = (w.dw[1][c1] * sz) * gscale;
+    r.gz = (w.sw[1][c1] * dz) * gscale;
+    return r;
+}
+
+template <int DIM>
+__device__ __forceinline__ bool window_in_patch(const Win<DIM> &w, const int *org) {
+    constexpr int P = Cfg<DIM>::P;
+    bool fits = true;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) fits = fits && (w.base[d] >= org[d]) && (w.base[d] + 3 <= org[d] + P);
+    return fits;
+}
+
+template <int DIM>
+__device__ __forceinline__ bool node_in_patch(const int *nijk, const int *org) {
+    constexpr int P = Cfg<DIM>::P;
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) in = in && nijk[d] >= org[d] && nijk[d] < org[d] + P;
+    return in;
+}
+
+// G2P accumulation of one node: u += S v dt, dv += S dv, L += v (x) G
+template <int DIM>
+__device__ __forceinline__ void g2p_acc(const double *vn, const double *dvn, double S, const double *G, double dt,
+                                        double *up, double *dvp, double (*Lp)[DIM]) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        up[d] = fma(S, vn[d] * dt, up[d]);
+        dvp[d] = fma(S, dvn[d], dvp[d]);
+    }
+#pragma unroll
+    for (int r = 0; r < DIM; r++)
+#pragma unroll
+        for (int c = 0; c < DIM; c++) Lp[r][c] = fma(vn[r], G[c], Lp[r][c]);
+}
+
+// node values (v, dv) from the global grid
+template <int DIM>
+__device__ __forceinline__ void fetch_node(const KArgs &A, const SimDev &s, long long gbase, const int *nijk,
+                                           double *vn, double *dvn) {
+    using G = GF<DIM>;
+    const long long gi = gbase + node_addr<DIM>(s, nijk);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        vn[d] = A.out[(G::OV + d) * A.nt + gi];
+        dvn[d] = A.out[(G::ODV + d) * A.nt + gi];
+    }
+}
+
+// movePoints / calculateStrainRate / updateDensity / updateStress for one point
+template <int DIM>
+__device__ __forceinline__ void advance_point(const BodyDev &B, double dt, PState<DIM> &q, const double *up,
+                                              const double *dvp, const double (*Lp)[DIM]) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        q.x[d] = q.x[d] + up[d];   // x += S^T u     (explicit_usl.cpp:279)
+        q.v[d] = q.v[d] + dvp[d];  // x_t += S^T dv  (explicit_usl.cpp:292, FLIP)
+    }
+    double tr = Lp[0][0] + Lp[1][1];
+    if (DIM == 3) tr = tr + Lp[DIM - 1][DIM - 1];
+    q.V = q.V * exp(dt * tr);  // explicit_usl.cpp:338
+    if (B.has_material) {
+        double L3[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+        for (int r = 0; r < DIM; r++)
+#pragma unroll
+            for (int c = 0; c < DIM; c++) L3[r][c] = Lp[r][c];
+        isolin_update<double>(B.twoG, B.lambda, dt, L3, q.T6);
+    }
+}
+
+// Particle pass over one particle block per workgroup (persistent over the block list).
+//   DO_G2P: gather the grid of the current step (LDS patch), update the point
+//   DO_P2G: map the (updated) point and scatter it into the next step's accumulators
+// NRM: accumulate contact normals gradS*m for bodies that are body 1 of a contact.
+// LDS planes are [field][padded node index] (Cfg::PSTR doubles per plane).
+template <int DIM, bool DO_G2P, bool DO_P2G, bool NRM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_particles(KArgs A, int want_L) {
+    constexpr int PATCH = Cfg<DIM>::PATCH, P = Cfg<DIM>::P, PSTR = Cfg<DIM>::PSTR, ZS = Cfg<DIM>::ZS;
+    constexpr int NACC = NRM ? GF<DIM>::NACC : 1 + 2 * DIM;
+    constexpr int NOUT = GF<DIM>::NOUT;
+    constexpr int NZ = DIM == 3 ? 3 : 1;
+    __shared__ double lacc[DO_P2G ? NACC * PSTR : 1];
+    __shared__ double gpat[DO_G2P ? NOUT * PSTR : 1];
     __shared__ int bb[2 * DIM];
     const int tid = threadIdx.x;
     const int nent = *A.npblk;
+    const double sscale = 1.0 / (double)(1 << DIM);
+#pragma unroll 1
     for (int e = blockIdx.x; e < nent; e += gridDim.x) {
         const PBlk ent = A.plist[e];
         const int body = uniform(ent.body);
         const int start = uniform(ent.start), count = uniform(ent.count);
         const BodyDev &B = A.bodies[body];
         const SimDev &s = A.sims[B.sim];
-        const bool b1 = B.is_b1 != 0;
+        const bool b1 = NRM && B.is_b1 != 0;
+        const long long gbase = B.grid_base;
+        const double dt = s.dt;
         int org[DIM];
         patch_origin<DIM>(s, uniform(ent.key) - B.pblk_off, org);
-        for (int i = tid; i < NACC * PATCH; i += blockDim.x) lacc[i] = 0.0;
-        if (tid < DIM) {
-            bb[tid] = 0x7fffffff;
-            bb[DIM + tid] = -0x7fffffff;
+        if (DO_P2G) {
+            for (int i = tid; i < NACC * PSTR; i += blockDim.x) lacc[i] = 0.0;
+            if (tid < DIM) {
+                bb[tid] = 0x7fffffff;
+                bb[DIM + tid] = -0x7fffffff;
+            }
+        }
+        if (DO_G2P) {
+            // stage the block's node patch: reads along node blocks, one field plane each
+            for (int i = tid; i < NOUT * PATCH; i += blockDim.x) {
+                const int f = i / PATCH, li = i - f * PATCH;
+                int nijk[DIM];
+                int r = li;
+                bool in = true;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    nijk[d] = org[d] + r % P;
+                    r /= P;
+                    in = in && nijk[d] >= 0 && nijk[d] <= s.N[d];
+                }
+                double v = 0.0;
+                if (in) v = A.out[f * A.nt + gbase + node_addr<DIM>(s, nijk)];
+                gpat[f * PSTR + patch_index<DIM>(nijk, org)] = v;
+            }
         }
         __syncthreads();
-        const long long gbase = B.grid_base;
+#pragma unroll 1
         for (int k = tid; k < count; k += blockDim.x) {
             const long long p = start + k;
             if (!A.act[p]) continue;
-            double x[DIM];
+            PState<DIM> q;
+            load_state<DIM>(A, p, q);
+            if (DO_G2P) {
+                Win<DIM> w;
+                bool cpdi;
+                const int mode = point_window<DIM>(s, q.x, q.ext, w, cpdi);
+                if (mode == MODE_LEFT) {  // not reachable after this step's map; kept for safety
+                    A.act[p] = 0;
+                    continue;
+                }
+                double up[DIM], dvp[DIM], Lp[DIM][DIM];
 #pragma unroll
-            for (int d = 0; d < DIM; d++) x[d] = A.P[(F::X + d) * A.np + p];
-            const double ext = A.P[F::EXT * A.np + p];
-            Win<DIM> w;
-            bool cpdi;
-            const int mode = point_window<DIM>(s, x, ext, w, cpdi);
-            if (mode == MODE_LEFT) {
-                A.act[p] = 0;
-                continue;
+                for (int d = 0; d < DIM; d++) {
+                    up[d] = dvp[d] = 0.0;
+#pragma unroll
+                    for (int c = 0; c < DIM; c++) Lp[d][c] = 0.0;
+                }
+                if (mode == MODE_SEP && window_in_patch<DIM>(w, org)) {
+                    // row-factored gather: per row, U = sum sw0 v, DV = sum sw0 dv, LX = sum dw0 v
+                    const double gscale = 1.0 / ((double)(1 << DIM) * 0.5 * q.ext);
+                    const int i0 = patch_index<DIM>(w.base, org);
+#pragma unroll
+                    for (int c2 = 0; c2 < NZ; c2++) {
+#pragma unroll
+                        for (int c1 = 0; c1 < 3; c1++) {
+                            const RowW<DIM> rw = row_weights<DIM>(w, c1, c2, sscale, gscale);
+                            double U[DIM], DV[DIM], LX[DIM];
+#pragma unroll
+                            for (int d = 0; d < DIM; d++) U[d] = DV[d] = LX[d] = 0.0;
+#pragma unroll
+                            for (int c0 = 0; c0 < 3; c0++) {
+                                const int i = i0 + c0 + P * c1 + ZS * c2;
+#pragma unroll
+                                for (int d = 0; d < DIM; d++) {
+                                    const double vn = gpat[d * PSTR + i];
+                                    const double dvn = gpat[(DIM + d) * PSTR + i];
+                                    U[d] = fma(w.sw[0][c0], vn, U[d]);
+                                    DV[d] = fma(w.sw[0][c0], dvn, DV[d]);
+                                    LX[d] = fma(w.dw[0][c0], vn, LX[d]);
+                                }
+                            }
+                            const double sdt = rw.Su * dt;
+#pragma unroll
+                            for (int d = 0; d < DIM; d++) {
+                                up[d] = fma(sdt, U[d], up[d]);
+                                dvp[d] = fma(rw.Su, DV[d], dvp[d]);
+                                Lp[d][0] = fma(rw.gx, LX[d], Lp[d][0]);
+                                Lp[d][1] = fma(rw.gy, U[d], Lp[d][1]);
+                                if (DIM == 3) Lp[d][DIM - 1] = fma(rw.gz, U[d], Lp[d][DIM - 1]);
+                            }
+                            // keep each row's LDS loads from being hoisted into earlier rows
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                    }
+                } else if (mode == MODE_SEP) {
+                    for_each_sep<DIM>(w, q.ext, [&](const int *off, double S, const double *G) {
+                        int nijk[DIM];
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) nijk[d] = w.base[d] + off[d];
+                        double vn[DIM], dvn[DIM];
+                        fetch_node<DIM>(A, s, gbase, nijk, vn, dvn);
+                        g2p_acc<DIM>(vn, dvn, S, G, dt, up, dvp, Lp);
+                    });
+                } else if (mode == MODE_TUP) {
+                    for_each_tuple<DIM>(s, q.x, q.ext, cpdi, [&](const int *nijk, double S, const double *G) {
+                        double vn[DIM], dvn[DIM];
+                        if (node_in_patch<DIM>(nijk, org)) {
+                            const int i = patch_index<DIM>(nijk, org);
+#pragma unroll
+                            for (int d = 0; d < DIM; d++) {
+                                vn[d] = gpat[d * PSTR + i];
+                                dvn[d] = gpat[(DIM + d) * PSTR + i];
+                            }
+                        } else {
+                            fetch_node<DIM>(A, s, gbase, nijk, vn, dvn);
+                        }
+                        g2p_acc<DIM>(vn, dvn, S, G, dt, up, dvp, Lp);
+                    });
+                }
+                advance_point<DIM>(B, dt, q, up, dvp, Lp);
+                store_state<DIM>(A, p, q);
+                if (want_L && A.Lout) {
+                    double L9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+                    for (int r = 0; r < DIM; r++)
+#pragma unroll
+                        for (int c = 0; c < DIM; c++) L9[3 * r + c] = Lp[r][c];
+#pragma unroll
+                    for (int kk = 0; kk < 9; kk++) A.Lout[kk * A.np + p] = L9[kk];
+                }
             }
-            if (mode == MODE_NONE) continue;
-            P2GVals<DIM> q;
-            q.m = A.P[F::MASS * A.np + p];
-            const double V = A.P[F::VOL * A.np + p];
+            if (DO_P2G) {
+                Win<DIM> w;
+                bool cpdi;
+                const int mode = point_window<DIM>(s, q.x, q.ext, w, cpdi);
+                if (mode == MODE_LEFT) {  // leaves the domain: deactivated (default_points.cpp:217-219)
+                    A.act[p] = 0;
+                    continue;
+                }
+                if (mode == MODE_NONE) continue;
+                P2GVals<DIM> pv;
+                p2g_payload<DIM>(q, pv);
+                auto scatter_global = [&](const int *nijk, double S, const double *G) {
+                    double vals[GF<DIM>::NACC];
+                    p2g_node_values<DIM>(pv, S, G, vals, b1);
+                    const long long gi = gbase + node_addr<DIM>(s, nijk);
 #pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                q.mv[d] = A.P[(F::V + d) * A.np + p] * q.m;
-                q.mg[d] = s.g[d] * q.m;
-            }
-            {
-                double T6[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-                for (int c = 0; c < Cfg<DIM>::NTS; c++) T6[c] = A.P[(F::T + c) * A.np + p];
-                const double Tf[3][3] = {{T6[0], T6[3], T6[4]}, {T6[3], T6[1], T6[5]}, {T6[4], T6[5], T6[2]}};
-#pragma unroll
-                for (int r = 0; r < DIM; r++)
-#pragma unroll
-                    for (int c = 0; c < DIM; c++) q.VT[r][c] = Tf[r][c] * V;  // T * v (explicit_usl.cpp:157-160)
-            }
-            if (mode == MODE_SEP) {
-                bool fits = true;
-#pragma unroll
-                for (int d = 0; d < DIM; d++)
-                    fits = fits && (w.base[d] >= org[d]) && (w.base[d] + w.cnt[d] <= org[d] + P);
-                const double gscale = 1.0 / ((double)(1 << DIM) * 0.5 * ext);
-                const double sscale = 1.0 / (double)(1 << DIM);
-                if (fits) {
+                    for (int f = 0; f < NACC; f++) {
+                        if (f >= 1 + 2 * DIM && !b1) continue;
+                        unsafeAtomicAdd(&A.acc[f * A.nt + gi], vals[f]);
+                    }
+                };
+                if (mode == MODE_SEP && window_in_patch<DIM>(w, org)) {
 #pragma unroll
                     for (int d = 0; d < DIM; d++) {
                         atomicMin(&bb[d], w.base[d]);
-                        atomicMax(&bb[DIM + d], w.base[d] + w.cnt[d] - 1);
+                        atomicMax(&bb[DIM + d], w.base[d] + 2);
                     }
-                } else {
-                    int lo[DIM], hi[DIM];
+                    // row-factored scatter: along a row every field is a * sw0 + b * dw0
+                    const double gscale = 1.0 / ((double)(1 << DIM) * 0.5 * q.ext);
+                    const int i0 = patch_index<DIM>(w.base, org);
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) {
-                        lo[d] = w.base[d];
-                        hi[d] = w.base[d] + w.cnt[d] - 1;
-                    }
-                    mark_range<DIM>(A, s, lo, hi);
-                }
+                    for (int c2 = 0; c2 < NZ; c2++) {
 #pragma unroll
-                for (int c2 = 0; c2 < (DIM == 3 ? 3 : 1); c2++) {
-                    if (DIM == 3 && c2 >= w.cnt[DIM - 1]) continue;
+                        for (int c1 = 0; c1 < 3; c1++) {
+                            const RowW<DIM> rw = row_weights<DIM>(w, c1, c2, sscale, gscale);
+                            const double am = pv.m * rw.Su;
+                            double ap[DIM], af0[DIM], af1[DIM];
 #pragma unroll
-                    for (int c1 = 0; c1 < 3; c1++) {
-                        if (c1 >= w.cnt[1]) continue;
+                            for (int d = 0; d < DIM; d++) {
+                                ap[d] = pv.mv[d] * rw.Su;
+                                af0[d] = -(pv.VT[d][0] * rw.gx);
+                                double t = pv.VT[d][1] * rw.gy;
+                                if (DIM == 3) t = fma(pv.VT[d][DIM - 1], rw.gz, t);
+                                af1[d] = -t;
+                            }
+                            const double an0 = pv.m * rw.gx, an1 = pv.m * rw.gy, an2 = pv.m * rw.gz;
 #pragma unroll
-                        for (int c0 = 0; c0 < 3; c0++) {
-                            if (c0 >= w.cnt[0]) continue;
-                            const double sz = DIM == 3 ? w.sw[DIM - 1][c2] : 1.0;
-                            const double dz = DIM == 3 ? w.dw[DIM - 1][c2] : 0.0;
-                            const double syz = w.sw[1][c1] * sz;
-                            const double S = w.sw[0][c0] * syz * sscale;
-                            double G[DIM];
-                            G[0] = w.dw[0][c0] * syz * gscale;
-                            G[1] = w.sw[0][c0] * (w.dw[1][c1] * sz) * gscale;
-                            if (DIM == 3) G[DIM - 1] = w.sw[0][c0] * (w.sw[1][c1] * dz) * gscale;
-                            double vals[NACC];
-                            p2g_node_values<DIM>(q, S, G, vals, b1);
-                            int nijk[DIM];
-                            nijk[0] = w.base[0] + c0;
-                            nijk[1] = w.base[1] + c1;
-                            if (DIM == 3) nijk[DIM - 1] = w.base[DIM - 1] + c2;
-                            if (fits) {
-                                int li = nijk[0] - org[0] + P * (nijk[1] - org[1]);
-                                if (DIM == 3) li += P * P * (nijk[DIM - 1] - org[DIM - 1]);
+                            for (int c0 = 0; c0 < 3; c0++) {
+                                const int i = i0 + c0 + P * c1 + ZS * c2;
+                                const double sx = w.sw[0][c0], dx = w.dw[0][c0];
+                                atomicAdd(&lacc[GF<DIM>::M * PSTR + i], am * sx);
 #pragma unroll
-                                for (int f = 0; f < NACC; f++) {
-                                    if (f >= 1 + 2 * DIM && !b1) continue;
-                                    atomicAdd(&lacc[f * PATCH + li], vals[f]);
+                                for (int d = 0; d < DIM; d++) {
+                                    atomicAdd(&lacc[(GF<DIM>::P + d) * PSTR + i], ap[d] * sx);
+                                    atomicAdd(&lacc[(GF<DIM>::F + d) * PSTR + i], fma(af0[d], dx, af1[d] * sx));
                                 }
-                            } else {
-                                const long long gi = gbase + node_addr<DIM>(s, nijk);
-#pragma unroll
-                                for (int f = 0; f < NACC; f++) {
-                                    if (f >= 1 + 2 * DIM && !b1) continue;
-                                    unsafeAtomicAdd(&A.acc[f * A.nt + gi], vals[f]);
+                                if (NRM && b1) {
+                                    atomicAdd(&lacc[(GF<DIM>::NRM + 0) * PSTR + i], an0 * dx);
+                                    atomicAdd(&lacc[(GF<DIM>::NRM + 1) * PSTR + i], an1 * sx);
+                                    if (DIM == 3) atomicAdd(&lacc[(GF<DIM>::NRM + DIM - 1) * PSTR + i], an2 * sx);
                                 }
                             }
                         }
                     }
-                }
-            } else {  // MODE_TUP
-                for_each_tuple<DIM>(s, x, ext, cpdi, [&](const int *nijk, double S, const double *G) {
-                    double vals[NACC];
-                    p2g_node_values<DIM>(q, S, G, vals, b1);
-                    bool in = true;
+                } else if (mode == MODE_SEP) {
+                    int lo[DIM], hi[DIM];
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) in = in && nijk[d] >= org[d] && nijk[d] < org[d] + P;
-                    if (in) {
-                        int li = nijk[0] - org[0] + P * (nijk[1] - org[1]);
-                        if (DIM == 3) li += P * P * (nijk[DIM - 1] - org[DIM - 1]);
-#pragma unroll
-                        for (int d = 0; d < DIM; d++) {
-                            atomicMin(&bb[d], nijk[d]);
-                            atomicMax(&bb[DIM + d], nijk[d]);
-                        }
-                        for (int f = 0; f < NACC; f++) {
-                            if (f >= 1 + 2 * DIM && !b1) continue;
-                            atomicAdd(&lacc[f * PATCH + li], vals[f]);
-                        }
-                    } else {
-                        const long long gi = gbase + node_addr<DIM>(s, nijk);
-                        for (int f = 0; f < NACC; f++) {
-                            if (f >= 1 + 2 * DIM && !b1) continue;
-                            unsafeAtomicAdd(&A.acc[f * A.nt + gi], vals[f]);
-                        }
-                        mark_range<DIM>(A, s, nijk, nijk);
+                    for (int d = 0; d < DIM; d++) {
+                        lo[d] = w.base[d];
+                        hi[d] = w.base[d] + 2;
                     }
-                });
+                    mark_range<DIM>(A, s, lo, hi);
+                    for_each_sep<DIM>(w, q.ext, [&](const int *off, double S, const double *G) {
+                        int nijk[DIM];
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) nijk[d] = w.base[d] + off[d];
+                        scatter_global(nijk, S, G);
+                    });
+                } else {  // MODE_TUP
+                    for_each_tuple<DIM>(s, q.x, q.ext, cpdi, [&](const int *nijk, double S, const double *G) {
+                        if (node_in_patch<DIM>(nijk, org)) {
+#pragma unroll
+                            for (int d = 0; d < DIM; d++) {
+                                atomicMin(&bb[d], nijk[d]);
+                                atomicMax(&bb[DIM + d], nijk[d]);
+                            }
+                            double vals[GF<DIM>::NACC];
+                            p2g_node_values<DIM>(pv, S, G, vals, b1);
+                            const int i = patch_index<DIM>(nijk, org);
+                            for (int f = 0; f < NACC; f++) {
+                                if (f >= 1 + 2 * DIM && !b1) continue;
+                                atomicAdd(&lacc[f * PSTR + i], vals[f]);
+                            }
+                        } else {
+                            mark_range<DIM>(A, s, nijk, nijk);
+                            scatter_global(nijk, S, G);
+                        }
+                    });
+                }
             }
         }
-        __syncthreads();
-        // flush the touched part of the patch
-        int lo[DIM], ex[DIM];
-        int total = 1;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            lo[d] = bb[d];
-            ex[d] = bb[DIM + d] >= bb[d] ? bb[DIM + d] - bb[d] + 1 : 0;  // empty when no LDS-path point
-            total *= ex[d];
-        }
-        for (int i = tid; i < total; i += blockDim.x) {
-            int r = i;
-            int nijk[DIM];
+        if (DO_P2G) {
+            __syncthreads();
+            // flush the touched part of the patch and mark its node blocks
+            int lo[DIM], ex[DIM];
+            int total = 1;
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
-                nijk[d] = lo[d] + r % ex[d];
-                r /= ex[d];
+                lo[d] = bb[d];
+                ex[d] = bb[DIM + d] >= bb[d] ? bb[DIM + d] - bb[d] + 1 : 0;  // empty: no LDS-path point
+                total *= ex[d];
             }
-            int li = nijk[0] - org[0] + P * (nijk[1] - org[1]);
-            if (DIM == 3) li += P * P * (nijk[DIM - 1] - org[DIM - 1]);
-            const long long gi = gbase + node_addr<DIM>(s, nijk);
+            for (int i = tid; i < total; i += blockDim.x) {
+                int r = i;
+                int nijk[DIM];
 #pragma unroll
-            for (int f = 0; f < NACC; f++) {
-                if (f >= 1 + 2 * DIM && !b1) continue;
-                const double v = lacc[f * PATCH + li];
-                if (v != 0.0) unsafeAtomicAdd(&A.acc[f * A.nt + gi], v);
+                for (int d = 0; d < DIM; d++) {
+                    nijk[d] = lo[d] + r % ex[d];
+                    r /= ex[d];
+                }
+                const int li = patch_index<DIM>(nijk, org);
+                const long long gi = gbase + node_addr<DIM>(s, nijk);
+#pragma unroll
+                for (int f = 0; f < NACC; f++) {
+                    if (f >= 1 + 2 * DIM && !b1) continue;
+                    const double v = lacc[f * PSTR + li];
+                    if (v != 0.0) unsafeAtomicAdd(&A.acc[f * A.nt + gi], v);
+                }
             }
-        }
-        if (tid == 0 && total > 0) {
-            int hi[DIM];
+            if (tid == 0 && total > 0) {
+                int hi[DIM];
 #pragma unroll
-            for (int d = 0; d < DIM; d++) hi[d] = lo[d] + ex[d] - 1;
-            mark_range<DIM>(A, s, lo, hi);
+                for (int d = 0; d < DIM; d++) hi[d] = lo[d] + ex[d] - 1;
+                mark_range<DIM>(A, s, lo, hi);
+            }
         }
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------------------
-// grid update: contact, boundary, moveGrid. One wavefront per node block.
+// grid update: body force, contact, boundary, moveGrid. One wavefront per node block.
 template <int DIM>
 __global__ __launch_bounds__(256) void k_grid(KArgs A) {
     using G = GF<DIM>;
@@ -300,21 +579,21 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
             long long t = lb;
             const int bx = (int)(t % s.NBn[0]);
             t /= s.NBn[0];
-            ijk[0] = (bx << L) + (lane & (NB - 1));
+            ijk[0] = (bx << L) + (lane & (NB - 1)) - 1;  // storage shifted by one node
             if (DIM == 3) {
                 const int by = (int)(t % s.NBn[1]);
                 const int bz = (int)(t / s.NBn[1]);
-                ijk[1] = (by << L) + ((lane >> L) & (NB - 1));
-                ijk[2] = (bz << L) + (lane >> (2 * L));
+                ijk[1] = (by << L) + ((lane >> L) & (NB - 1)) - 1;
+                ijk[2] = (bz << L) + (lane >> (2 * L)) - 1;
             } else {
-                ijk[1] = ((int)t << L) + (lane >> L);
+                ijk[1] = ((int)t << L) + (lane >> L) - 1;
             }
         }
         bool valid = true;
         bool wall = false;
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
-            valid = valid && ijk[d] <= s.N[d];
+            valid = valid && ijk[d] >= 0 && ijk[d] <= s.N[d];
             wall = wall || ijk[d] == 0 || ijk[d] == s.N[d];
         }
         const long long addr = lb * 64 + lane;
@@ -334,7 +613,8 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
 #pragma unroll
                     for (int d = 0; d < DIM; d++) {
                         p[b][d] = A.acc[(G::P + d) * A.nt + gi];
-                        f[b][d] = A.acc[(G::F + d) * A.nt + gi];
+                        // nodal body force m_i g (default_driver.cpp:109-119 via explicit_usl.cpp:149-154)
+                        f[b][d] = fma(m[b], s.g[d], A.acc[(G::F + d) * A.nt + gi]);
                         A.acc[(G::P + d) * A.nt + gi] = 0.0;
                         A.acc[(G::F + d) * A.nt + gi] = 0.0;
                     }
@@ -446,130 +726,6 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
         unsigned long long tot = active_nodes;
         for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
         if (lane == 0 && tot) atomicAdd(&A.stats[0], tot);
-    }
-}
-
-// ---------------------------------------------------------------------------------------
-// G2P: movePoints, calculateStrainRate, updateDensity, updateStress
-template <int DIM>
-__device__ __forceinline__ void g2p_node(const KArgs &A, long long gi, double S, const double *G, double dt,
-                                         double *up, double *dvp, double (*Lp)[DIM]) {
-    using GG = GF<DIM>;
-    double v[DIM];
-#pragma unroll
-    for (int d = 0; d < DIM; d++) {
-        v[d] = A.out[(GG::OV + d) * A.nt + gi];
-        const double dv = A.out[(GG::ODV + d) * A.nt + gi];
-        up[d] = fma(S, v[d] * dt, up[d]);
-        dvp[d] = fma(S, dv, dvp[d]);
-    }
-#pragma unroll
-    for (int r = 0; r < DIM; r++)
-#pragma unroll
-        for (int c = 0; c < DIM; c++) Lp[r][c] = fma(v[r], G[c], Lp[r][c]);
-}
-
-template <int DIM>
-__global__ __launch_bounds__(256) void k_g2p(KArgs A, int want_L) {
-    using F = PF<DIM>;
-    const int tid = threadIdx.x;
-    const int nent = *A.npblk;
-    for (int e = blockIdx.x; e < nent; e += gridDim.x) {
-        const PBlk ent = A.plist[e];
-        const int body = uniform(ent.body);
-        const int start = uniform(ent.start), count = uniform(ent.count);
-        const BodyDev &B = A.bodies[body];
-        const SimDev &s = A.sims[B.sim];
-        const long long gbase = B.grid_base;
-        const double dt = s.dt;
-        for (int k = tid; k < count; k += blockDim.x) {
-            const long long p = start + k;
-            if (!A.act[p]) continue;
-            double x[DIM];
-#pragma unroll
-            for (int d = 0; d < DIM; d++) x[d] = A.P[(F::X + d) * A.np + p];
-            const double ext = A.P[F::EXT * A.np + p];
-            Win<DIM> w;
-            bool cpdi;
-            const int mode = point_window<DIM>(s, x, ext, w, cpdi);
-            if (mode == MODE_LEFT) {  // cannot happen after the P2G of this step; kept for safety
-                A.act[p] = 0;
-                continue;
-            }
-            double up[DIM], dvp[DIM], Lp[DIM][DIM];
-#pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                up[d] = dvp[d] = 0.0;
-#pragma unroll
-                for (int c = 0; c < DIM; c++) Lp[d][c] = 0.0;
-            }
-            if (mode == MODE_SEP) {
-                const double gscale = 1.0 / ((double)(1 << DIM) * 0.5 * ext);
-                const double sscale = 1.0 / (double)(1 << DIM);
-#pragma unroll
-                for (int c2 = 0; c2 < (DIM == 3 ? 3 : 1); c2++) {
-                    if (DIM == 3 && c2 >= w.cnt[DIM - 1]) continue;
-#pragma unroll
-                    for (int c1 = 0; c1 < 3; c1++) {
-                        if (c1 >= w.cnt[1]) continue;
-#pragma unroll
-                        for (int c0 = 0; c0 < 3; c0++) {
-                            if (c0 >= w.cnt[0]) continue;
-                            const double sz = DIM == 3 ? w.sw[DIM - 1][c2] : 1.0;
-                            const double dz = DIM == 3 ? w.dw[DIM - 1][c2] : 0.0;
-                            const double syz = w.sw[1][c1] * sz;
-                            const double S = w.sw[0][c0] * syz * sscale;
-                            double G[DIM];
-                            G[0] = w.dw[0][c0] * syz * gscale;
-                            G[1] = w.sw[0][c0] * (w.dw[1][c1] * sz) * gscale;
-                            if (DIM == 3) G[DIM - 1] = w.sw[0][c0] * (w.sw[1][c1] * dz) * gscale;
-                            int nijk[DIM];
-                            nijk[0] = w.base[0] + c0;
-                            nijk[1] = w.base[1] + c1;
-                            if (DIM == 3) nijk[DIM - 1] = w.base[DIM - 1] + c2;
-                            g2p_node<DIM>(A, gbase + node_addr<DIM>(s, nijk), S, G, dt, up, dvp, Lp);
-                        }
-                    }
-                }
-            } else if (mode == MODE_TUP) {
-                for_each_tuple<DIM>(s, x, ext, cpdi, [&](const int *nijk, double S, const double *G) {
-                    g2p_node<DIM>(A, gbase + node_addr<DIM>(s, nijk), S, G, dt, up, dvp, Lp);
-                });
-            }
-            // movePoints: x += S^T u; x_t += S^T dv (FLIP)
-#pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                A.P[(F::X + d) * A.np + p] = x[d] + up[d];
-                A.P[(F::V + d) * A.np + p] = A.P[(F::V + d) * A.np + p] + dvp[d];
-            }
-            // updateDensity: v *= exp(dt * trace(L))
-            double tr = Lp[0][0] + Lp[1][1];
-            if (DIM == 3) tr = tr + Lp[DIM - 1][DIM - 1];
-            A.P[F::VOL * A.np + p] = A.P[F::VOL * A.np + p] * exp(dt * tr);
-            // updateStress
-            if (B.has_material) {
-                double L3[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll
-                for (int r = 0; r < DIM; r++)
-#pragma unroll
-                    for (int c = 0; c < DIM; c++) L3[r][c] = Lp[r][c];
-                double T6[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-                for (int c = 0; c < Cfg<DIM>::NTS; c++) T6[c] = A.P[(F::T + c) * A.np + p];
-                isolin_update<double>(B.twoG, B.lambda, dt, L3, T6);
-#pragma unroll
-                for (int c = 0; c < Cfg<DIM>::NTS; c++) A.P[(F::T + c) * A.np + p] = T6[c];
-            }
-            if (want_L && A.Lout) {
-                double L9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-                for (int r = 0; r < DIM; r++)
-#pragma unroll
-                    for (int c = 0; c < DIM; c++) L9[3 * r + c] = Lp[r][c];
-#pragma unroll
-                for (int k = 0; k < 9; k++) A.Lout[k * A.np + p] = L9[k];
-            }
-        }
     }
 }
 

@@ -2,7 +2,11 @@
 //
 // A particle block is a PB^DIM group of dual cells (the cell of x/h - 1/2, so every point
 // of a block touches nodes inside one (PB+2)^DIM window). Sorting keeps the particles of
-// a block contiguous, which is what lets k_p2g accumulate a whole block in one LDS patch.
+// a block contiguous, which is what lets k_particles accumulate a whole block in one LDS
+// patch. Inside a block the key orders points by the octant of their position within the
+// dual cell first: the 64 consecutive points a wavefront takes then sit in ~64 different
+// dual cells, so their LDS atomics hit different nodes (no same-address serialisation)
+// while the loads stay coalesced.
 // The reference has no counterpart: it scatters straight from point order through COO
 // tuples (src/algebra/mpm_sparse.cpp:30-44).
 //
@@ -27,7 +31,7 @@ __device__ __forceinline__ int body_of_index(const KArgs &A, long long o) {
     return lo;
 }
 
-__device__ __forceinline__ int body_of_key(const KArgs &A, long long key) {
+__device__ __forceinline__ int body_of_block(const KArgs &A, long long key) {
     int lo = 0, hi = A.nbody - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -55,14 +59,18 @@ __global__ __launch_bounds__(256) void k_bin_count(KArgs A, unsigned *count, int
         for (int d = 0; d < DIM; d++) x[d] = A.P[(F::X + d) * A.np + p];
         if (in_domain<DIM>(s, x)) {
             long long lk = 0, stride = 1;
+            int oct = 0;
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
-                int c = (int)floor(x[d] * s.invh[d] - 0.5) + 1;  // dual cell, shifted to >= 0
+                const double t = x[d] * s.invh[d] - 0.5;
+                const double ft = floor(t);
+                int c = (int)ft + 1;  // dual cell, shifted to >= 0
                 c = c < 0 ? 0 : (c > s.N[d] ? s.N[d] : c);
                 lk += (long long)(c / PB) * stride;
                 stride *= s.PNB[d];
+                oct |= (t - ft >= 0.5 ? 1 : 0) << d;
             }
-            key = B.pblk_off + lk;
+            key = (B.pblk_off + lk) * (1 << DIM) + oct;
         } else {
             A.act[p] = 0;  // leaves the domain: deactivated (default_points.cpp:217-219)
         }
@@ -122,21 +130,33 @@ __global__ __launch_bounds__(1024) void k_scan_add(unsigned *out, const unsigned
         if (i + k < n) out[i + k] += add;
 }
 
+// one entry per non-empty particle block (keys are block * 2^DIM + octant); clears the
+// histogram for the next sort
 __global__ __launch_bounds__(256) void k_make_plist(KArgs A, unsigned *count, const unsigned *off, PBlk *plist,
-                                                    int *npblk, long long nkeys) {
+                                                    int *npblk, long long nkeys, int noct) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nkeys) return;
-    const unsigned c = count[k];
-    if (c > 0 && k < nkeys - 1) {
-        const int pos = atomicAdd(npblk, 1);
-        PBlk e;
-        e.key = (int)k;
-        e.start = (int)off[k];
-        e.count = (int)c;
-        e.body = body_of_key(A, k);
-        plist[pos] = e;
+    if (k < nkeys - 1 && k % noct == 0) {
+        const unsigned b0 = off[k], b1 = off[k + noct];
+        if (b1 > b0) {
+            const int pos = atomicAdd(npblk, 1);
+            PBlk e;
+            e.key = (int)(k / noct);
+            e.start = (int)b0;
+            e.count = (int)(b1 - b0);
+            e.body = body_of_block(A, k / noct);
+            plist[pos] = e;
+        }
     }
-    count[k] = 0u;
+}
+
+__global__ void k_clear_u64(unsigned long long *p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_clear(unsigned *count, long long n) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) count[k] = 0u;
 }
 
 template <int NF>
@@ -170,10 +190,18 @@ __global__ __launch_bounds__(256) void k_unsort_act(long long np, const int8_t *
 }
 
 __global__ __launch_bounds__(256) void k_count_active(long long np, const int8_t *act, unsigned long long *out) {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool a = p < np && act[p];
-    const unsigned long long b = __ballot(a);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(out, (unsigned long long)__popcll(b));
+    __shared__ unsigned wsum[4];
+    unsigned c = 0;
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (long long)gridDim.x * blockDim.x)
+        c += act[p] ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += wsum[i];
+        if (t) atomicAdd(out, t);
+    }
 }
 
 }  // namespace mpm

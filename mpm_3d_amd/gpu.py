@@ -26,8 +26,11 @@ SYMBOLS = (
     "mpm_gpu_create", "mpm_gpu_create_batch", "mpm_gpu_upload", "mpm_gpu_step", "mpm_gpu_download",
     "mpm_gpu_download_aux", "mpm_gpu_connectivity", "mpm_gpu_set_sort_interval", "mpm_gpu_set_timing",
     "mpm_gpu_get_timing", "mpm_gpu_stats", "mpm_gpu_synchronize", "mpm_gpu_destroy",
-    "mpm_gpu_last_error", "mpm_gpu_footprint",
+    "mpm_gpu_last_error", "mpm_gpu_footprint", "mpm_gpu_host_weights", "mpm_gpu_copy_probe",
+    "mpm_gpu_set_option",
 )
+MPM_OPT_SORT_INTERVAL = 1
+MPM_OPT_FUSED_PASS = 2
 
 
 class MpmGpuError(RuntimeError):
@@ -76,6 +79,7 @@ def load_library(path: str = LIB_PATH):
     lib.mpm_gpu_connectivity.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)]
     lib.mpm_gpu_set_sort_interval.argtypes = [vp, C.c_int]
     lib.mpm_gpu_set_timing.argtypes = [vp, C.c_int]
+    lib.mpm_gpu_set_option.argtypes = [vp, C.c_int, C.c_int]
     lib.mpm_gpu_get_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     lib.mpm_gpu_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.mpm_gpu_synchronize.argtypes = [vp]
@@ -84,6 +88,10 @@ def load_library(path: str = LIB_PATH):
     lib.mpm_gpu_last_error.argtypes = []
     lib.mpm_gpu_footprint.restype = C.c_int64
     lib.mpm_gpu_footprint.argtypes = [C.POINTER(SimDesc), C.c_int, C.c_int]
+    lib.mpm_gpu_copy_probe.argtypes = [C.c_int64, C.c_int, _dp]
+    lib.mpm_gpu_host_weights.restype = C.c_int
+    lib.mpm_gpu_host_weights.argtypes = [C.POINTER(GridDesc), _dp, C.c_double, C.POINTER(C.c_int32), _dp, _dp,
+                                         C.c_int]
     for name in SYMBOLS:
         getattr(lib, name)
     _lib = lib
@@ -116,7 +124,7 @@ def _grid_desc(dim, L, N, cpdi_spec, contact_spec, gravity):
 class GpuSim:
     """One handle on the GPU holding one or more simulations (``workloads``)."""
 
-    def __init__(self, workloads, precision=MPM_PREC_FP64, sort_interval=16):
+    def __init__(self, workloads, precision=MPM_PREC_FP64, sort_interval=16, fused=True):
         if not isinstance(workloads, (list, tuple)):
             workloads = [workloads]
         self.lib = load_library()
@@ -149,6 +157,7 @@ class GpuSim:
         _check(self.lib.mpm_gpu_create_batch(sims, len(self.workloads), precision, C.byref(h)))
         self.h = h
         _check(self.lib.mpm_gpu_set_sort_interval(self.h, sort_interval))
+        _check(self.lib.mpm_gpu_set_option(self.h, MPM_OPT_FUSED_PASS, int(bool(fused))))
         for gb, (si, bi) in enumerate(self.body_index):
             self.upload(gb, self.workloads[si].bodies[bi])
 
@@ -236,6 +245,37 @@ class GpuSim:
             self.close()
         except Exception:
             pass
+
+
+def host_weights(dim, L, N, x, extent, cpdi_spec=1):
+    """The kernels' shape-function tuples for one point, evaluated on the host (no GPU).
+
+    Returns (node_ijk (k, dim), S (k,), G (k, dim)), or None when x is outside the domain."""
+    lib = load_library()
+    g = _grid_desc(dim, L, N, cpdi_spec, 1, (0.0, 0.0, 0.0))
+    xa = np.zeros(3)
+    xa[:dim] = x
+    cap = 64
+    nid = np.zeros(3 * cap, np.int32)
+    S = np.zeros(cap)
+    G = np.zeros(3 * cap)
+    k = lib.mpm_gpu_host_weights(C.byref(g), xa.ctypes.data_as(_dp), float(extent),
+                                 nid.ctypes.data_as(C.POINTER(C.c_int32)), S.ctypes.data_as(_dp),
+                                 G.ctypes.data_as(_dp), cap)
+    if k == -100:
+        return None
+    if k < 0:
+        _check(k)
+    k = min(k, cap)
+    return nid.reshape(cap, 3)[:k, :dim], S[:k], G.reshape(cap, 3)[:k, :dim]
+
+
+def copy_probe(nbytes=1 << 30, reps=20):
+    """HBM GB/s of a plain device copy (read + write) on the current GPU."""
+    lib = load_library()
+    g = C.c_double()
+    _check(lib.mpm_gpu_copy_probe(int(nbytes), int(reps), C.byref(g)))
+    return g.value
 
 
 def stress6_to9(T6):

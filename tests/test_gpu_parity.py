@@ -139,3 +139,19 @@ def test_sort_interval_invariance():
     for bi in range(2):
         pa, pb = a.points(bi), b.points(bi)
         assert np.max(np.abs(pa["x"] - pb["x"])) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["collision2D", "collision3D"])
+def test_fused_and_split_passes_agree(name):
+    """One fused G2P+P2G pass per step and separate passes give the same state to rounding,
+    and both match the oracle."""
+    w = example_workload(name)
+    ora = oracle_of(w)
+    a = gpu_of(w, fused=True)
+    b = gpu_of(w, fused=False)
+    ora.step(w.dt, 60)
+    a.step(w.dt, 30)
+    a.step(w.dt, 30)
+    b.step(w.dt, 60)
+    compare_states(w, ora, a, 60 * w.dt, label=f"{name} fused")
+    compare_states(w, ora, b, 60 * w.dt, label=f"{name} split")
