@@ -69,7 +69,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--sort-interval", type=int, default=16)
-    ap.add_argument("--split", action="store_true", help="separate G2P and P2G passes instead of one fused pass")
+    ap.add_argument("--fused", action="store_true", help="one fused G2P+P2G pass per step instead of two passes")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 10 launches) used to calibrate counters")
     args = ap.parse_args()
@@ -87,7 +87,7 @@ def main():
     hbm_copy = copy_probe(1 << 30, 20)
     w = W.column_3d(wide=True)
     print(f"[bench] workload {w.name}: {w.n_points} points, grid {w.N}", file=sys.stderr, flush=True)
-    sim = GpuSim(w, sort_interval=args.sort_interval, fused=not args.split)
+    sim = GpuSim(w, sort_interval=args.sort_interval, fused=args.fused)
     if args.warmup:
         sim.step(w.dt, args.warmup)
     sim.synchronize()
@@ -136,7 +136,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "points": int(w.n_points), "grid": list(w.N), "dt": w.dt,
                    "form": "plain FP64", "sort_interval": args.sort_interval,
-                   "particle_pass": "split" if args.split else "fused"},
+                   "particle_pass": "fused" if args.fused else "split"},
         "roofline": roofline,
         "step_hbm_fraction": round(step_bytes / wall / 1e9 / HBM_PEAK_GBS, 4),
         "hbm_copy_gbps": round(hbm_copy, 1),

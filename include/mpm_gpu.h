@@ -129,8 +129,8 @@ int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
 
 /* Execution options (speed only; results agree to rounding):
  *   MPM_OPT_SORT_INTERVAL  as mpm_gpu_set_sort_interval
- *   MPM_OPT_FUSED_PASS     1 (default): one particle pass per step doing G2P and the next
- *                          step's P2G; 0: separate G2P and P2G passes */
+ *   MPM_OPT_FUSED_PASS     1: one particle pass per step doing G2P and the next step's
+ *                          P2G; 0 (default): separate G2P and P2G passes */
 #define MPM_OPT_SORT_INTERVAL 1
 #define MPM_OPT_FUSED_PASS 2
 int mpm_gpu_set_option(mpm_gpu_handle *h, int key, int value);

@@ -29,9 +29,12 @@ ARCH = "gfx950"
 # -ffp-contract=off: no implicit multiply-add fusion, so the connectivity arithmetic
 # (corner positions, element indices) rounds exactly like the reference; hot loops use
 # explicit fma(). Packed FP32 instructions are disabled for all device code.
+# The atomic optimizer's DPP strategy reduces same-address atomics (the LDS bounding-box
+# min/max of a particle block) with a few DPP steps; the default iterative strategy
+# loops over every active lane on the scalar unit, thousands of SALU ops per point.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-    "-ffp-contract=off", "-munsafe-fp-atomics",
+    "-ffp-contract=off", "-munsafe-fp-atomics", "-mllvm", "-amdgpu-atomic-optimizer-strategy=DPP",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]

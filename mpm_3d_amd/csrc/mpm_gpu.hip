@@ -204,7 +204,7 @@ struct mpm_gpu_handle {
     long long stage_elems = 0;
     int par = 0;
     int sort_interval = 16;
-    int fused = 1;  // 1: one particle pass per step (G2P then P2G); 0: two passes
+    int fused = 0;  // 1: one particle pass per step (G2P then P2G); 0: two passes (default: faster on gfx950)
     bool sims_dirty = true;
     std::vector<char> uploaded;
     std::vector<std::vector<double>> x0;  // per body, dim planes (for displacement)

@@ -124,7 +124,7 @@ def _grid_desc(dim, L, N, cpdi_spec, contact_spec, gravity):
 class GpuSim:
     """One handle on the GPU holding one or more simulations (``workloads``)."""
 
-    def __init__(self, workloads, precision=MPM_PREC_FP64, sort_interval=16, fused=True):
+    def __init__(self, workloads, precision=MPM_PREC_FP64, sort_interval=16, fused=False):
         if not isinstance(workloads, (list, tuple)):
             workloads = [workloads]
         self.lib = load_library()
