@@ -5,8 +5,11 @@ h = 7.8125 mm, 2^3 points per cell jittered +-0.25 cell, E = 1e7 Pa, nu = 0.3,
 rho = 1500, dt = 1e-5, gravity, CartesianBox walls), synthetic and seeded.
 
 A step is one full ExplicitUSL step over all points with the state resident in HBM:
-sort (every 16 steps), map + P2G, grid update, G2P + stress. `value` = active points x
-timed steps / wall time of the timed steps.
+sort (every 16 steps), map + P2G, grid update, G2P + stress. Both forms run on the same
+input: the plain FP64 form and the precision form (FP32 particle state and stress, FP64
+nodal accumulation, grid update and contact normals). `value` is the precision form's
+active points x timed steps / wall time of the timed steps; `forms` holds both and
+`fp32_vs_fp64` their ratio.
 
 The JSON line also carries
   roofline      the dominant kernel's algorithmic HBM bytes per launch / its average launch
@@ -28,14 +31,22 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-updates/s, 16M-pt 3D (FP32 vs FP64); batched solves/s + p99 latency"
 HBM_PEAK_GBS = 8000.0
 
-# algorithmic bytes (3D, FP64 form) per launch, as (bytes per active point, bytes per
-# active node); the per-kernel table is in DESIGN.md
+# algorithmic bytes per launch, 3D, as (bytes per active point, bytes per active node);
+# the per-kernel table is in DESIGN.md
 BYTES = {
-    "pass": (224, 104),         # read x3 v3 T6 V m extent, write x3 v3 T6 V; read v3 dv3, write m p3 f3
-    "grid": (0, 104),           # read m p3 f3, write v3 dv3
-    "p2g": (120, 56),           # read x3 v3 T6 V m extent; write m p3 f3
-    "g2p": (216, 48),           # read x3 v3 T6 V extent, write x3 v3 T6 V; read v3 dv3
-    "step": (224, 208),         # one particle pass + one grid update
+    "fp64": {
+        "pass": (224, 104),     # read x3 v3 T6 V m extent, write x3 v3 T6 V; read v3 dv3, write m p3 f3
+        "grid": (0, 104),       # read m p3 f3, write v3 dv3
+        "p2g": (120, 56),       # read x3 v3 T6 V m extent; write m p3 f3
+        "g2p": (216, 48),       # read x3 v3 T6 V extent, write x3 v3 T6 V; read v3 dv3
+        "step": (224, 208),     # one particle pass + one grid update
+    },
+    "fp32": {
+        "grid": (0, 80),        # read m p3 f3 (FP64), write v3 dv3 (FP32)
+        "p2g": (68, 56),        # read cell q3 v3 T6 lnJ V0 m extent (4 B each); write m p3 f3 (FP64)
+        "g2p": (116, 24),       # read cell q3 v3 T6 lnJ extent, write cell q3 v3 T6 lnJ; read v3 dv3 (FP32)
+        "step": (124, 160),     # each plane read once and the updated ones written once
+    },
 }
 CLASSES = {0: "pass", 1: "grid", 2: "sort", 3: "p2g", 4: "g2p"}
 
@@ -61,33 +72,11 @@ def cpu_baseline(threads):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--sort-interval", type=int, default=16)
-    ap.add_argument("--fused", action="store_true", help="one fused G2P+P2G pass per step instead of two passes")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="only run the plain copy kernel (1 GiB, 10 launches) used to calibrate counters")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is "
-              "design-only in this build", file=sys.stderr)
-        return 2
-
-    from mpm_3d_amd import workloads as W
-    from mpm_3d_amd.gpu import GpuSim, copy_probe
-
-    if args.copy_probe:
-        print(json.dumps({"copy_probe_gbps": copy_probe(1 << 30, 10), "bytes_per_launch": 2 * (1 << 30)}))
-        return 0
-    hbm_copy = copy_probe(1 << 30, 20)
-    w = W.column_3d(wide=True)
-    print(f"[bench] workload {w.name}: {w.n_points} points, grid {w.N}", file=sys.stderr, flush=True)
-    sim = GpuSim(w, sort_interval=args.sort_interval, fused=args.fused)
+def run_form(w, form, args):
+    """Time `args.steps` steps of one form; returns its summary."""
+    from mpm_3d_amd.gpu import MPM_PREC_FP32, MPM_PREC_FP64, GpuSim
+    prec = MPM_PREC_FP32 if form == "fp32" else MPM_PREC_FP64
+    sim = GpuSim(w, precision=prec, sort_interval=args.sort_interval, fused=args.fused)
     if args.warmup:
         sim.step(w.dt, args.warmup)
     sim.synchronize()
@@ -97,52 +86,93 @@ def main():
     sim.synchronize()
     wall = time.perf_counter() - t0
     active_nodes, active_points = sim.stats()
-    print(f"[bench] {args.steps} steps in {wall:.3f} s; active points {active_points}, nodes {active_nodes}",
-          file=sys.stderr, flush=True)
-
     kern = {}
     for cls, name in CLASSES.items():
         n, ms = sim.timing(cls)
         if n:
             kern[name] = {"launches": n, "avg_ms": ms / n, "total_ms": ms}
-    dom = max((k for k in kern if k in BYTES), key=lambda k: kern[k]["total_ms"])
-    bp, bn = BYTES[dom]
+    sim.close()
+    print(f"[bench] {form}: {args.steps} steps in {wall:.3f} s; active points {active_points}, "
+          f"nodes {active_nodes}", file=sys.stderr, flush=True)
+    table = BYTES[form]
+    dom = max((k for k in kern if k in table), key=lambda k: kern[k]["total_ms"])
+    bp, bn = table[dom]
     bytes_per_launch = bp * active_points + bn * active_nodes
     achieved = bytes_per_launch / (kern[dom]["avg_ms"] * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_column_3d_16M.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{form}_column_3d_16M.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-    roofline = {
-        "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-        "algorithmic_bytes_per_launch": bytes_per_launch,
+    step_bytes = (table["step"][0] * active_points + table["step"][1] * active_nodes) * args.steps
+    return {
+        "value": active_points * args.steps / wall,
+        "ms_per_step": wall / args.steps * 1e3,
+        "roofline": {
+            "bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+        },
+        "step_hbm_fraction": round(step_bytes / wall / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels": kern,
     }
-    step_bytes = (BYTES["step"][0] * active_points + BYTES["step"][1] * active_nodes) * args.steps
-    value = active_points * args.steps / wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--sort-interval", type=int, default=16)
+    ap.add_argument("--fused", action="store_true", help="one fused G2P+P2G pass per step (FP64 form only)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the plain copy kernel (1 GiB, 10 launches) used to calibrate counters")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is "
+              "design-only in this build", file=sys.stderr)
+        return 2
+
+    from mpm_3d_amd import workloads as W
+    from mpm_3d_amd.gpu import copy_probe
+
+    if args.copy_probe:
+        print(json.dumps({"copy_probe_gbps": copy_probe(1 << 30, 10), "bytes_per_launch": 2 * (1 << 30)}))
+        return 0
+    hbm_copy = copy_probe(1 << 30, 20)
+    w = W.column_3d(wide=True)
+    print(f"[bench] workload {w.name}: {w.n_points} points, grid {w.N}", file=sys.stderr, flush=True)
+    forms = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
+    res = {f: run_form(w, f, args) for f in forms}
+    main_form = "fp32" if "fp32" in res else "fp64"
+    r = res[main_form]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": r["value"],
         "unit": "particle-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 state, f64 accumulation" if main_form == "fp32" else "f64",
         "data": "synthetic",
         "config": {"workload": w.name, "points": int(w.n_points), "grid": list(w.N), "dt": w.dt,
-                   "form": "plain FP64", "sort_interval": args.sort_interval,
-                   "particle_pass": "fused" if args.fused else "split"},
-        "roofline": roofline,
-        "step_hbm_fraction": round(step_bytes / wall / 1e9 / HBM_PEAK_GBS, 4),
+                   "form": "precision (FP32 state, FP64 accumulation)" if main_form == "fp32" else "plain FP64",
+                   "sort_interval": args.sort_interval, "particle_pass": "fused" if args.fused else "split"},
+        "roofline": r["roofline"],
+        "step_hbm_fraction": r["step_hbm_fraction"],
         "hbm_copy_gbps": round(hbm_copy, 1),
-        "kernels": kern,
+        "kernels": r["kernels"],
+        "forms": res,
     }
-    sim.close()
+    if len(res) == 2:
+        out["fp32_vs_fp64"] = round(res["fp32"]["value"] / res["fp64"]["value"], 3)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         print(f"[bench] cpu baseline with {threads} threads", file=sys.stderr, flush=True)

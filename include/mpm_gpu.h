@@ -123,8 +123,9 @@ int mpm_gpu_download_aux(mpm_gpu_handle *h, int body, double *u, double *L);
  * (default_points.cpp:196-330, cartesian.cpp:219-253). */
 int mpm_gpu_connectivity(mpm_gpu_handle *h, int body, int32_t *corner_elem, uint8_t *code);
 
-/* Points re-sorted by grid block every `interval` steps (default 16; 0 = only at the
- * start of each mpm_gpu_step call). Affects speed only. */
+/* Points re-sorted by grid block every `interval` steps, counted across mpm_gpu_step
+ * calls (default 16; 0 = at the start of every mpm_gpu_step call). An upload always
+ * forces a sort before the next step. Affects speed only. */
 int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
 
 /* Execution options (speed only; results agree to rounding):

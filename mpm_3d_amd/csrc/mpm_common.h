@@ -3,10 +3,15 @@
 //
 // Layouts (all structure-of-arrays, one plane per component):
 //   particles  P[f * np + slot]      f = x[DIM], v[DIM], T[NTS], V, m, extent (FP64 form)
-//              slots are kept sorted by particle block (see mpm_sort.hip) and carry the
+//              Pw[f * np + slot]     f = cell, q[DIM], v[DIM], T[NTS], lnJ, V0, m, extent
+//                                    (precision form: 4-byte planes; x = h (cell + q 2^-32)
+//                                    with q a 32-bit fixed-point offset inside the cell,
+//                                    V = V0 exp(lnJ), the rest FP32)
+//              slots are kept sorted by particle block (see mpm_sort.h) and carry the
 //              point's original global index in pid[slot]
 //   grid       acc[f * nt + node]    accumulators m, p[DIM], f[DIM], n[DIM] (contact normal)
-//              out[f * nt + node]    v[DIM] (velocity after the update), dv[DIM]
+//              out[f * nt + node]    v[DIM] (velocity after the update), dv[DIM]; FP64,
+//                                    or FP32 (out32) in the precision form
 //              node = body.grid_base + 64 * node_block + local; a node block is 4x4x4 (3D)
 //              or 8x8 (2D) nodes so one wavefront owns one block.
 //   stress     NTS = 6 in 3D (xx, yy, zz, xy, xz, yz), 4 in 2D (xx, yy, zz, xy)
@@ -42,6 +47,7 @@ struct SimDev {
     double g[3];
     double dt;
     int cpdi_spec, contact_spec;
+    float hf[3], invhf[3];  // h and 1/h rounded to FP32 (precision form)
 };
 
 struct BodyDev {
@@ -95,6 +101,41 @@ template <int DIM> struct PF {  // particle field planes (FP64 form)
     static constexpr int VOL = 2 * DIM + NTS, MASS = VOL + 1, EXT = VOL + 2, N = VOL + 3;
 };
 
+// Precision-form particle planes: 4-byte words, plane 0 the packed cell index.
+template <int DIM> struct PW {
+    static constexpr int CELL = 0, Q = 1, V = 1 + DIM, T = 1 + 2 * DIM, NTS = Cfg<DIM>::NTS;
+    static constexpr int LNJ = 1 + 2 * DIM + NTS, V0 = LNJ + 1, MASS = LNJ + 2, EXT = LNJ + 3, N = LNJ + 4;
+};
+
+// Packed cell index of the precision form: (i + 1) | (j + 1) << B0 | (k + 1) << (B0 + B1),
+// so cells -1..2^B-2 fit (a point may sit one cell outside the grid for the step between
+// leaving the domain and being deactivated).
+template <int DIM> struct CellBits;
+template <> struct CellBits<3> { static constexpr int B[3] = {11, 11, 10}; };
+template <> struct CellBits<2> { static constexpr int B[3] = {16, 16, 0}; };
+
+template <int DIM>
+__host__ __device__ __forceinline__ uint32_t pack_cell(const int *c) {
+    uint32_t w = 0;
+    int sh = 0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        w |= (uint32_t)(c[d] + 1) << sh;
+        sh += CellBits<DIM>::B[d];
+    }
+    return w;
+}
+
+template <int DIM>
+__host__ __device__ __forceinline__ void unpack_cell(uint32_t w, int *c) {
+    int sh = 0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        c[d] = (int)((w >> sh) & ((1u << CellBits<DIM>::B[d]) - 1u)) - 1;
+        sh += CellBits<DIM>::B[d];
+    }
+}
+
 template <int DIM> struct GF {  // grid field planes
     static constexpr int M = 0, P = 1, F = 1 + DIM, NRM = 1 + 2 * DIM, NACC = 1 + 3 * DIM;
     static constexpr int OV = 0, ODV = DIM, NOUT = 2 * DIM;
@@ -110,8 +151,10 @@ struct KArgs {
     long long np;
     int8_t *act;
     int *pid;
+    uint32_t *Pw;  // precision-form particle planes
     double *acc;
     double *out;
+    float *out32;  // precision-form grid output
     long long nt;
     unsigned *nflag;
     int *nlist;
@@ -194,10 +237,10 @@ __host__ __device__ __forceinline__ long long which_element(const SimDev &s, con
 // slots no corner reaches carry exact zeros. The fixed 3^D stencil keeps the node loops
 // free of per-lane branches. Points whose corners span more than 2 elements in some
 // dimension (extent > h) take the per-tuple path.
-template <int DIM> struct Win {
+template <int DIM, class R = double> struct Win {
     int base[DIM];
-    double sw[DIM][3];
-    double dw[DIM][3];
+    R sw[DIM][3];
+    R dw[DIM][3];
 };
 
 // cpdi_out tells the tuple path whether the point uses its corners (CPDI on) or not.

@@ -85,6 +85,8 @@ int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
                 s.invh[d] = 1.0 / s.h[d];
                 s.NoverL[d] = (double)g.N[d] / g.L[d];
                 s.ftol[d] = std::ldexp((double)g.N[d], -48);
+                s.hf[d] = (float)s.h[d];
+                s.invhf[d] = (float)s.invh[d];
                 s.NBn[d] = (int)ceil_div(g.N[d] + 3, NB);  // nodes -1..N+1
                 s.PNB[d] = (int)ceil_div(g.N[d] + 1, PB);
                 nblk *= s.NBn[d];
@@ -157,14 +159,16 @@ int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
 }
 
 long long footprint_bytes(const Layout &Lo, int precision) {
-    (void)precision;
-    const int nf = Lo.dim == 3 ? PF<3>::N : PF<2>::N;
+    const bool f32 = precision == MPM_PREC_FP32;
+    const int nf = f32 ? (Lo.dim == 3 ? PW<3>::N : PW<2>::N) : (Lo.dim == 3 ? PF<3>::N : PF<2>::N);
+    const int esz = f32 ? 4 : 8;
     const int nacc = Lo.dim == 3 ? GF<3>::NACC : GF<2>::NACC;
     const int nout = Lo.dim == 3 ? GF<3>::NOUT : GF<2>::NOUT;
     long long b = 0;
-    b += 2 * Lo.np * (nf * 8 + 1 + 4);     // particle planes, double-buffered, + act + pid
+    b += 2 * Lo.np * (nf * esz + 1 + 4);   // particle planes, double-buffered, + act + pid
     b += Lo.np * (4 + 4);                  // sort key + rank
-    b += Lo.nt * (nacc + nout) * 8;        // grid
+    b += Lo.np * 9 * 8;                    // velocity gradient of the last step
+    b += Lo.nt * (nacc * 8 + nout * esz);  // grid
     b += Lo.nblk_total * (4 + 2 * 4);      // node-block flags + two lists
     b += Lo.nkeys * (4 + 4);               // histogram, offsets
     b += Lo.npblk_total * sizeof(PBlk);    // block list
@@ -185,11 +189,13 @@ struct mpm_gpu_handle {
     SimDev *d_sims = nullptr;
     BodyDev *d_bodies = nullptr;
     ContactDev *d_contacts = nullptr;
-    double *P = nullptr, *P2 = nullptr;
+    double *P = nullptr, *P2 = nullptr;     // plain form planes
+    uint32_t *Pw = nullptr, *Pw2 = nullptr;  // precision form planes
     int8_t *act = nullptr, *act2 = nullptr;
     int *pid = nullptr, *pid2 = nullptr;
     int *pkey = nullptr, *prank = nullptr;
     double *acc = nullptr, *out = nullptr;
+    float *out32 = nullptr;
     unsigned *nflag = nullptr;
     int *nlist = nullptr;
     int *ncount = nullptr;
@@ -204,6 +210,8 @@ struct mpm_gpu_handle {
     long long stage_elems = 0;
     int par = 0;
     int sort_interval = 16;
+    bool sorted = false;  // slots are in particle-block order (cleared by upload)
+    int since_sort = 0;   // steps advanced since the last sort
     int fused = 0;  // 1: one particle pass per step (G2P then P2G); 0: two passes (default: faster on gfx950)
     bool sims_dirty = true;
     std::vector<char> uploaded;
@@ -229,8 +237,10 @@ KArgs kargs(mpm_gpu_handle *h) {
     A.np = h->lo.np;
     A.act = h->act;
     A.pid = h->pid;
+    A.Pw = h->Pw;
     A.acc = h->acc;
     A.out = h->out;
+    A.out32 = h->out32;
     A.nt = h->lo.nt;
     A.nflag = h->nflag;
     A.nlist = h->nlist;
@@ -286,7 +296,7 @@ int dalloc(T **p, long long n) {
     return MPM_OK;
 }
 
-template <int DIM>
+template <int DIM, class PT>
 int launch_sort(mpm_gpu_handle *h) {
     const long long np = h->lo.np, nk = h->lo.nkeys;
     KArgs A = kargs(h);
@@ -294,7 +304,7 @@ int launch_sort(mpm_gpu_handle *h) {
     int rc = timer_begin(h, KC_SORT, t);
     if (rc) return rc;
     const int gp = (int)std::max(1ll, ceil_div(np, 256));
-    hipLaunchKernelGGL(k_bin_count<DIM>, dim3(gp), dim3(256), 0, h->stream, A, h->bcount, h->pkey, h->prank,
+    hipLaunchKernelGGL((k_bin_count<DIM, PT>), dim3(gp), dim3(256), 0, h->stream, A, h->bcount, h->pkey, h->prank,
                        nk - 1);
     // exclusive scan of the histogram, multi-level
     {
@@ -323,9 +333,15 @@ int launch_sort(mpm_gpu_handle *h) {
     hipLaunchKernelGGL(k_make_plist, dim3((unsigned)ceil_div(nk, 256)), dim3(256), 0, h->stream, A, h->bcount,
                        h->boff, h->plist, h->npblk, nk, 1 << DIM);
     hipLaunchKernelGGL(k_clear, dim3((unsigned)ceil_div(nk, 256)), dim3(256), 0, h->stream, h->bcount, nk);
-    hipLaunchKernelGGL(k_bin_scatter<PF<DIM>::N>, dim3(gp), dim3(256), 0, h->stream, np, h->P, h->P2, h->act,
-                       h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff);
-    std::swap(h->P, h->P2);
+    if constexpr (PT::FP32) {
+        hipLaunchKernelGGL((k_bin_scatter<uint32_t, PW<DIM>::N>), dim3(gp), dim3(256), 0, h->stream, np, h->Pw, h->Pw2,
+                           h->act, h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff);
+        std::swap(h->Pw, h->Pw2);
+    } else {
+        hipLaunchKernelGGL((k_bin_scatter<double, PF<DIM>::N>), dim3(gp), dim3(256), 0, h->stream, np, h->P, h->P2,
+                           h->act, h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff);
+        std::swap(h->P, h->P2);
+    }
     std::swap(h->act, h->act2);
     std::swap(h->pid, h->pid2);
     HIP_TRY(hipGetLastError());
@@ -334,26 +350,33 @@ int launch_sort(mpm_gpu_handle *h) {
 
 constexpr int PASS_GRID = 2048;  // persistent workgroups for the particle passes
 
-template <int DIM, bool G2P, bool P2G>
+template <int DIM, class PT, bool G2P, bool P2G>
 void launch_particles(mpm_gpu_handle *h, int want_L) {
     KArgs A = kargs(h);
     if (h->lo.any_b1)
-        hipLaunchKernelGGL((k_particles<DIM, G2P, P2G, true>), dim3(PASS_GRID), dim3(256), 0, h->stream, A, want_L);
+        hipLaunchKernelGGL((k_particles<DIM, G2P, P2G, true, PT>), dim3(PASS_GRID), dim3(256), 0, h->stream, A, want_L);
     else
-        hipLaunchKernelGGL((k_particles<DIM, G2P, P2G, false>), dim3(PASS_GRID), dim3(256), 0, h->stream, A, want_L);
+        hipLaunchKernelGGL((k_particles<DIM, G2P, P2G, false, PT>), dim3(PASS_GRID), dim3(256), 0, h->stream, A,
+                           want_L);
 }
 
 // Step sequence with the list parity `par`: a P2G half marks node blocks into list[par];
 // k_grid(par) consumes it and clears list[par^1] for the next P2G.
 //   sort; P2G(par); { grid(par); par^=1; [sort]; G2P+P2G(par) } ...; grid(par); par^=1; G2P
-template <int DIM>
+template <int DIM, class PT>
 int launch_step(mpm_gpu_handle *h, int nsteps) {
     Timer t{};
-    int rc = launch_sort<DIM>(h);
-    if (rc) return rc;
+    int rc = MPM_OK;
+    // sort when the order is stale: after an upload, when the interval has passed, or at
+    // every call with interval 0; a driver stepping one step per call pays no extra sort
+    if (!h->sorted || h->sort_interval == 0 || h->since_sort >= h->sort_interval) {
+        if ((rc = launch_sort<DIM, PT>(h))) return rc;
+        h->sorted = true;
+        h->since_sort = 0;
+    }
     rc = timer_begin(h, KC_P2G, t);
     if (rc) return rc;
-    launch_particles<DIM, false, true>(h, 0);
+    launch_particles<DIM, PT, false, true>(h, 0);
     if ((rc = timer_end(h, t))) return rc;
     for (int k = 0; k < nsteps; k++) {
         const bool last = k == nsteps - 1;
@@ -365,28 +388,30 @@ int launch_step(mpm_gpu_handle *h, int nsteps) {
             } else {
                 Ag.stats = nullptr;
             }
-            hipLaunchKernelGGL(k_grid<DIM>, dim3(PASS_GRID), dim3(256), 0, h->stream, Ag);
+            hipLaunchKernelGGL((k_grid<DIM, typename PT::GO>), dim3(PASS_GRID), dim3(256), 0, h->stream, Ag);
         }
         if ((rc = timer_end(h, t))) return rc;
         h->par ^= 1;
+        h->since_sort++;
         if (last) {
             if ((rc = timer_begin(h, KC_G2P, t))) return rc;
-            launch_particles<DIM, true, false>(h, 1);
+            launch_particles<DIM, PT, true, false>(h, 1);
             if ((rc = timer_end(h, t))) return rc;
         } else {
-            if (h->sort_interval > 0 && (k + 1) % h->sort_interval == 0) {
-                if ((rc = launch_sort<DIM>(h))) return rc;
+            if (h->sort_interval > 0 && h->since_sort >= h->sort_interval) {
+                if ((rc = launch_sort<DIM, PT>(h))) return rc;
+                h->since_sort = 0;
             }
-            if (h->fused) {
+            if (h->fused && !PT::FP32) {
                 if ((rc = timer_begin(h, KC_PASS, t))) return rc;
-                launch_particles<DIM, true, true>(h, 0);
+                launch_particles<DIM, Prec64, true, true>(h, 0);
                 if ((rc = timer_end(h, t))) return rc;
             } else {
                 if ((rc = timer_begin(h, KC_G2P, t))) return rc;
-                launch_particles<DIM, true, false>(h, 0);
+                launch_particles<DIM, PT, true, false>(h, 0);
                 if ((rc = timer_end(h, t))) return rc;
                 if ((rc = timer_begin(h, KC_P2G, t))) return rc;
-                launch_particles<DIM, false, true>(h, 0);
+                launch_particles<DIM, PT, false, true>(h, 0);
                 if ((rc = timer_end(h, t))) return rc;
             }
         }
@@ -396,7 +421,17 @@ int launch_step(mpm_gpu_handle *h, int nsteps) {
 }
 
 int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
-    if (precision != MPM_PREC_FP64) return fail(MPM_ERR_ARG, "precision form %d not available in this build", precision);
+    if (precision != MPM_PREC_FP64 && precision != MPM_PREC_FP32)
+        return fail(MPM_ERR_ARG, "unknown precision form %d", precision);
+    if (precision == MPM_PREC_FP32) {
+        // the packed cell index holds cells -1 .. 2^bits - 2 per dimension
+        for (const SimDev &s : lo.sims)
+            for (int d = 0; d < lo.dim; d++) {
+                const int bits = lo.dim == 3 ? CellBits<3>::B[d] : CellBits<2>::B[d];
+                if (s.N[d] + 2 > (1 << bits))
+                    return fail(MPM_ERR_ARG, "precision form: N[%d] = %d exceeds %d cells", d, s.N[d], (1 << bits) - 2);
+            }
+    }
     mpm_gpu_handle *h = new mpm_gpu_handle();
     h->lo = std::move(lo);
     h->precision = precision;
@@ -414,14 +449,22 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     if (e != hipSuccess) return cleanup(fail(MPM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)));
     const Layout &L = h->lo;
     const int dim = L.dim;
-    const int nf = dim == 3 ? PF<3>::N : PF<2>::N;
+    const bool f32 = precision == MPM_PREC_FP32;
+    const int nf = f32 ? (dim == 3 ? PW<3>::N : PW<2>::N) : (dim == 3 ? PF<3>::N : PF<2>::N);
     const int nacc = dim == 3 ? GF<3>::NACC : GF<2>::NACC;
     const int nout = dim == 3 ? GF<3>::NOUT : GF<2>::NOUT;
     TRY(dalloc(&h->d_sims, (long long)L.sims.size()));
     TRY(dalloc(&h->d_bodies, (long long)L.bodies.size()));
     TRY(dalloc(&h->d_contacts, std::max<long long>(1, (long long)L.contacts.size())));
-    TRY(dalloc(&h->P, L.np * nf));
-    TRY(dalloc(&h->P2, L.np * nf));
+    if (f32) {
+        TRY(dalloc(&h->Pw, L.np * nf));
+        TRY(dalloc(&h->Pw2, L.np * nf));
+        TRY(dalloc(&h->out32, L.nt * nout));
+    } else {
+        TRY(dalloc(&h->P, L.np * nf));
+        TRY(dalloc(&h->P2, L.np * nf));
+        TRY(dalloc(&h->out, L.nt * nout));
+    }
     TRY(dalloc(&h->act, L.np));
     TRY(dalloc(&h->act2, L.np));
     TRY(dalloc(&h->pid, L.np));
@@ -429,7 +472,6 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     TRY(dalloc(&h->pkey, L.np));
     TRY(dalloc(&h->prank, L.np));
     TRY(dalloc(&h->acc, L.nt * nacc));
-    TRY(dalloc(&h->out, L.nt * nout));
     TRY(dalloc(&h->nflag, L.nblk_total));
     TRY(dalloc(&h->nlist, 2 * L.nblk_total));
     TRY(dalloc(&h->ncount, 2));
@@ -449,12 +491,16 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     }
     // zero state: accumulators must start at zero, flags clear, counters zero
     e = hipMemsetAsync(h->acc, 0, sizeof(double) * L.nt * nacc, h->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(h->out, 0, sizeof(double) * L.nt * nout, h->stream);
+    if (e == hipSuccess)
+        e = f32 ? hipMemsetAsync(h->out32, 0, sizeof(float) * L.nt * nout, h->stream)
+                : hipMemsetAsync(h->out, 0, sizeof(double) * L.nt * nout, h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->nflag, 0, sizeof(unsigned) * std::max(1ll, L.nblk_total), h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->ncount, 0, sizeof(int) * 2, h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->bcount, 0, sizeof(unsigned) * L.nkeys, h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->npblk, 0, sizeof(int), h->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(h->P, 0, sizeof(double) * std::max(1ll, L.np * nf), h->stream);
+    if (e == hipSuccess)
+        e = f32 ? hipMemsetAsync(h->Pw, 0, sizeof(uint32_t) * std::max(1ll, L.np * nf), h->stream)
+                : hipMemsetAsync(h->P, 0, sizeof(double) * std::max(1ll, L.np * nf), h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->act, 0, std::max(1ll, L.np), h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->stats, 0, sizeof(unsigned long long) * 4, h->stream);
     if (e == hipSuccess)
@@ -528,12 +574,16 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
     // The sorted slots of other bodies are interleaved with this body's, so an upload
     // writes the staging buffer in original order and re-sorts from there: bring every
     // body back to original order first.
-    const int nf = dim == 3 ? PF<3>::N : PF<2>::N;
+    const bool f32 = h->precision == MPM_PREC_FP32;
+    const int nf = f32 ? (dim == 3 ? PW<3>::N : PW<2>::N) : (dim == 3 ? PF<3>::N : PF<2>::N);
     HIP_TRY(hipStreamSynchronize(h->stream));
     {
-        // unsort P -> P2 (all planes), act -> act2, pid -> identity
+        // unsort the planes -> second buffer, act -> act2, pid -> identity
         const int gp = (int)ceil_div(np, 256);
-        hipLaunchKernelGGL(k_unsort, dim3(gp), dim3(256), 0, h->stream, np, nf, h->P, np, h->P2, h->pid);
+        if (f32)
+            hipLaunchKernelGGL(k_unsort<uint32_t>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->Pw, np, h->Pw2, h->pid);
+        else
+            hipLaunchKernelGGL(k_unsort<double>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->P, np, h->P2, h->pid);
         int *tmp_act = nullptr;
         HIP_TRY(hipMalloc(&tmp_act, sizeof(int) * np));
         hipLaunchKernelGGL(k_unsort_act, dim3(gp), dim3(256), 0, h->stream, np, h->act, tmp_act, h->pid);
@@ -548,11 +598,59 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
         std::vector<int> ids(np);
         for (long long i = 0; i < np; i++) ids[i] = (int)i;
         HIP_TRY(hipMemcpyAsync(h->pid2, ids.data(), sizeof(int) * np, hipMemcpyHostToDevice, h->stream));
+        const int NTS = dim == 3 ? 6 : 4;
+        if (f32) {
+            // precision form: cell + 32-bit fixed-point offset (x = h (cell + q 2^-32)),
+            // lnJ = 0 with V0 = V, FP32 fields
+            const SimDev &s = h->lo.sims[B.sim];
+            std::vector<uint32_t> pl((size_t)nf * n);
+            float *pf = reinterpret_cast<float *>(pl.data());
+            const int FQ = 1, FV = 1 + dim, FT = 1 + 2 * dim, FLNJ = 1 + 2 * dim + NTS;
+            for (long long i = 0; i < n; i++) {
+                int c[3] = {0, 0, 0};
+                for (int d = 0; d < dim; d++) {
+                    const double t = x[(long long)d * n + i] / s.h[d];
+                    double cf = std::floor(t);
+                    double fr = t - cf;  // in [0, 1)
+                    if (!(cf >= -1.0)) {  // far outside (or NaN): deactivated by the first sort
+                        cf = -1.0;
+                        fr = 0.0;
+                    } else if (cf > (double)s.N[d]) {
+                        cf = (double)s.N[d];
+                        fr = 0.5;
+                    }
+                    double qf = std::nearbyint(std::ldexp(fr, 32));
+                    if (qf >= 4294967296.0) {  // rounds up to the next node
+                        qf = 0.0;
+                        cf += 1.0;
+                    }
+                    c[d] = (int)cf;
+                    pl[(FQ + d) * n + i] = (uint32_t)qf;
+                    pf[(FV + d) * n + i] = (float)v[(long long)d * n + i];
+                }
+                pl[i] = dim == 3 ? pack_cell<3>(c) : pack_cell<2>(c);
+                for (int k = 0; k < NTS; k++) pf[(FT + k) * n + i] = T ? (float)T[(long long)k * n + i] : 0.0f;
+                pf[FLNJ * n + i] = 0.0f;
+                pf[(FLNJ + 1) * n + i] = (float)V[i];
+                pf[(FLNJ + 2) * n + i] = (float)m[i];
+                pf[(FLNJ + 3) * n + i] = (float)extent[i];
+            }
+            for (int f = 0; f < nf; f++)
+                HIP_TRY(hipMemcpyAsync(h->Pw2 + (long long)f * np + B.p_off, pl.data() + (size_t)f * n,
+                                       sizeof(uint32_t) * n, hipMemcpyHostToDevice, h->stream));
+            HIP_TRY(hipStreamSynchronize(h->stream));
+            std::swap(h->Pw, h->Pw2);
+            std::swap(h->act, h->act2);
+            std::swap(h->pid, h->pid2);
+            h->x0[body].assign(x, x + (long long)dim * n);
+            h->uploaded[body] = 1;
+            h->sorted = false;
+            return MPM_OK;
+        }
         // this body's planes
         using F3 = PF<3>;
         using F2 = PF<2>;
         const int FX = 0, FV = dim, FT = 2 * dim;
-        const int NTS = dim == 3 ? 6 : 4;
         const int FVOL = dim == 3 ? F3::VOL : F2::VOL, FMASS = FVOL + 1, FEXT = FVOL + 2;
         auto put = [&](int f, const double *src) -> hipError_t {
             return hipMemcpyAsync(h->P2 + (long long)f * np + B.p_off, src, sizeof(double) * n, hipMemcpyHostToDevice,
@@ -582,6 +680,7 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
     }
     h->x0[body].assign(x, x + (long long)dim * n);
     h->uploaded[body] = 1;
+    h->sorted = false;
     return MPM_OK;
 }
 
@@ -629,7 +728,9 @@ int mpm_gpu_step(mpm_gpu_handle *h, double dt, int nsteps) {
         HIP_TRY(hipMalloc(&h->Lout, sizeof(double) * 9 * std::max(1ll, h->lo.np)));
         HIP_TRY(hipMemsetAsync(h->Lout, 0, sizeof(double) * 9 * std::max(1ll, h->lo.np), h->stream));
     }
-    const int rc = h->lo.dim == 3 ? launch_step<3>(h, nsteps) : launch_step<2>(h, nsteps);
+    const bool f32 = h->precision == MPM_PREC_FP32;
+    const int rc = h->lo.dim == 3 ? (f32 ? launch_step<3, Prec32>(h, nsteps) : launch_step<3, Prec64>(h, nsteps))
+                                  : (f32 ? launch_step<2, Prec32>(h, nsteps) : launch_step<2, Prec64>(h, nsteps));
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (h->timing) {
@@ -656,28 +757,58 @@ int mpm_gpu_download(mpm_gpu_handle *h, int body, double *x, double *v, double *
     const int dim = h->lo.dim;
     const long long n = B.n, np = h->lo.np;
     if (n == 0) return MPM_OK;
-    const int nf = dim == 3 ? PF<3>::N : PF<2>::N;
+    const bool f32 = h->precision == MPM_PREC_FP32;
+    const int nf = f32 ? (dim == 3 ? PW<3>::N : PW<2>::N) : (dim == 3 ? PF<3>::N : PF<2>::N);
     HIP_TRY(hipStreamSynchronize(h->stream));
     int rc = ensure_stage(h, np * nf);
     if (rc) return rc;
     const int gp = (int)ceil_div(np, 256);
-    hipLaunchKernelGGL(k_unsort, dim3(gp), dim3(256), 0, h->stream, np, nf, h->P, np, h->stage, h->pid);
-    HIP_TRY(hipGetLastError());
+    if (f32) {
+        uint32_t *st = reinterpret_cast<uint32_t *>(h->stage);
+        hipLaunchKernelGGL(k_unsort<uint32_t>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->Pw, np, st, h->pid);
+        HIP_TRY(hipGetLastError());
+        std::vector<uint32_t> pl((size_t)nf * n);
+        for (int f = 0; f < nf; f++)
+            HIP_TRY(hipMemcpyAsync(pl.data() + (size_t)f * n, st + (long long)f * np + B.p_off, sizeof(uint32_t) * n,
+                                   hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        const float *pf = reinterpret_cast<const float *>(pl.data());
+        const SimDev &s = h->lo.sims[B.sim];
+        const int NTS = dim == 3 ? 6 : 4;
+        const int FQ = 1, FV = 1 + dim, FT = 1 + 2 * dim, FLNJ = 1 + 2 * dim + NTS;
+        for (long long i = 0; i < n; i++) {
+            int c[3] = {0, 0, 0};
+            if (dim == 3) unpack_cell<3>(pl[i], c);
+            else unpack_cell<2>(pl[i], c);
+            for (int d = 0; d < dim; d++) {
+                if (x) x[(long long)d * n + i] = s.h[d] * ((double)c[d] + std::ldexp((double)pl[(FQ + d) * n + i], -32));
+                if (v) v[(long long)d * n + i] = pf[(FV + d) * n + i];
+            }
+            if (T)
+                for (int k = 0; k < 6; k++) T[(long long)k * n + i] = k < NTS ? (double)pf[(FT + k) * n + i] : 0.0;
+            if (V) V[i] = (double)pf[(FLNJ + 1) * n + i] * std::exp((double)pf[FLNJ * n + i]);
+        }
+    } else {
+        hipLaunchKernelGGL(k_unsort<double>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->P, np, h->stage, h->pid);
+        HIP_TRY(hipGetLastError());
+    }
     auto get = [&](int f, double *dst) -> hipError_t {
         return hipMemcpyAsync(dst, h->stage + (long long)f * np + B.p_off, sizeof(double) * n, hipMemcpyDeviceToHost,
                               h->stream);
     };
     const int FX = 0, FV = dim, FT = 2 * dim, NTS = dim == 3 ? 6 : 4;
     const int FVOL = 2 * dim + NTS;
-    for (int d = 0; d < dim; d++) {
-        if (x) HIP_TRY(get(FX + d, x + (long long)d * n));
-        if (v) HIP_TRY(get(FV + d, v + (long long)d * n));
+    if (!f32) {
+        for (int d = 0; d < dim; d++) {
+            if (x) HIP_TRY(get(FX + d, x + (long long)d * n));
+            if (v) HIP_TRY(get(FV + d, v + (long long)d * n));
+        }
+        if (T) {
+            for (int c = 0; c < NTS; c++) HIP_TRY(get(FT + c, T + (long long)c * n));
+            for (int c = NTS; c < 6; c++) std::memset(T + (long long)c * n, 0, sizeof(double) * n);
+        }
+        if (V) HIP_TRY(get(FVOL, V));
     }
-    if (T) {
-        for (int c = 0; c < NTS; c++) HIP_TRY(get(FT + c, T + (long long)c * n));
-        for (int c = NTS; c < 6; c++) std::memset(T + (long long)c * n, 0, sizeof(double) * n);
-    }
-    if (V) HIP_TRY(get(FVOL, V));
     if (active) {
         int *tmp = nullptr;
         HIP_TRY(hipMalloc(&tmp, sizeof(int) * np));
@@ -711,7 +842,7 @@ int mpm_gpu_download_aux(mpm_gpu_handle *h, int body, double *u, double *L) {
         int rc = ensure_stage(h, np * 9);
         if (rc) return rc;
         const int gp = (int)ceil_div(np, 256);
-        hipLaunchKernelGGL(k_unsort, dim3(gp), dim3(256), 0, h->stream, np, 9, h->Lout, np, h->stage, h->pid);
+        hipLaunchKernelGGL(k_unsort<double>, dim3(gp), dim3(256), 0, h->stream, np, 9, h->Lout, np, h->stage, h->pid);
         HIP_TRY(hipGetLastError());
         for (int k = 0; k < 9; k++)
             HIP_TRY(hipMemcpyAsync(L + (long long)k * n, h->stage + (long long)k * np + B.p_off, sizeof(double) * n,
@@ -736,10 +867,14 @@ int mpm_gpu_connectivity(mpm_gpu_handle *h, int body, int32_t *corner_elem, uint
     HIP_TRY(hipMalloc(&d_code, np));
     KArgs A = kargs(h);
     const int gp = (int)ceil_div(np, 256);
-    if (dim == 3)
-        hipLaunchKernelGGL(k_connectivity<3>, dim3(gp), dim3(256), 0, h->stream, A, d_elem, d_code);
-    else
-        hipLaunchKernelGGL(k_connectivity<2>, dim3(gp), dim3(256), 0, h->stream, A, d_elem, d_code);
+    const bool f32 = h->precision == MPM_PREC_FP32;
+    if (dim == 3) {
+        if (f32) hipLaunchKernelGGL((k_connectivity<3, Prec32>), dim3(gp), dim3(256), 0, h->stream, A, d_elem, d_code);
+        else hipLaunchKernelGGL((k_connectivity<3, Prec64>), dim3(gp), dim3(256), 0, h->stream, A, d_elem, d_code);
+    } else {
+        if (f32) hipLaunchKernelGGL((k_connectivity<2, Prec32>), dim3(gp), dim3(256), 0, h->stream, A, d_elem, d_code);
+        else hipLaunchKernelGGL((k_connectivity<2, Prec64>), dim3(gp), dim3(256), 0, h->stream, A, d_elem, d_code);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(corner_elem, d_elem + B.p_off * nc, sizeof(int) * n * nc, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipMemcpyAsync(code, d_code + B.p_off, n, hipMemcpyDeviceToHost, h->stream));
@@ -888,8 +1023,8 @@ int mpm_gpu_synchronize(mpm_gpu_handle *h) {
 int mpm_gpu_destroy(mpm_gpu_handle *h) {
     if (!h) return MPM_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void *ptrs[] = {h->d_sims, h->d_bodies, h->d_contacts, h->P, h->P2, h->act, h->act2, h->pid, h->pid2,
-                    h->pkey, h->prank, h->acc, h->out, h->nflag, h->nlist, h->ncount, h->bcount, h->boff,
+    void *ptrs[] = {h->d_sims, h->d_bodies, h->d_contacts, h->P, h->P2, h->Pw, h->Pw2, h->act, h->act2, h->pid,
+                    h->pid2, h->pkey, h->prank, h->acc, h->out, h->out32, h->nflag, h->nlist, h->ncount, h->bcount, h->boff,
                     h->plist, h->npblk, h->Lout, h->stats, h->stage};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);

@@ -1,6 +1,7 @@
-// mpm_kernels.h -- the explicit USL step as HIP kernels for gfx950 (FP64 form).
+// mpm_kernels.h -- the explicit USL step as HIP kernels for gfx950, for both particle-state
+// forms (mpm_prec.h).
 //
-// One step (explicit_usl.cpp:48-87) is two launches over resident HBM state:
+// One step (explicit_usl.cpp:48-87) is two or three launches over resident HBM state:
 //   k_grid       per touched node: body force m g, ContactHuang (contact_huang.cpp:104-197),
 //                CartesianBox (cartesian_box.cpp:73-85), moveGrid (explicit_usl.cpp:233-264);
 //                one wavefront per 64-node block; consumes and re-zeroes the accumulators.
@@ -11,7 +12,8 @@
 //                the map (default_points.cpp:196-330) and P2G (explicit_usl.cpp:112-190)
 //                with the contact-normal scatter gradS*m (contact_huang.cpp:80-99) into an
 //                LDS accumulator patch (FP64 LDS atomics), flushed to the grid with FP64
-//                global atomics. The first step of a call runs the P2G half alone, the
+//                global atomics. The G2P and P2G halves run as separate passes (default)
+//                or fused into one; the first step of a call runs the P2G half alone, the
 //                last the G2P half alone.
 //
 // The nodal body force: every point carries b = g (default_driver.cpp:109-119), so
@@ -20,6 +22,7 @@
 #pragma once
 
 #include "mpm_common.h"
+#include "mpm_prec.h"
 
 namespace mpm {
 
@@ -68,54 +71,19 @@ __host__ __device__ __forceinline__ int patch_index(const int *nijk, const int *
     return li;
 }
 
-// one point's state (FP64 form)
-template <int DIM> struct PState {
-    double x[DIM], v[DIM], T6[6], V, m, ext;
-};
-
-template <int DIM>
-__device__ __forceinline__ void load_state(const KArgs &A, long long p, PState<DIM> &q) {
-    using F = PF<DIM>;
-#pragma unroll
-    for (int d = 0; d < DIM; d++) {
-        q.x[d] = A.P[(F::X + d) * A.np + p];
-        q.v[d] = A.P[(F::V + d) * A.np + p];
-    }
-#pragma unroll
-    for (int c = 0; c < 6; c++) q.T6[c] = 0.0;
-#pragma unroll
-    for (int c = 0; c < F::NTS; c++) q.T6[c] = A.P[(F::T + c) * A.np + p];
-    q.V = A.P[F::VOL * A.np + p];
-    q.m = A.P[F::MASS * A.np + p];
-    q.ext = A.P[F::EXT * A.np + p];
-}
-
-template <int DIM>
-__device__ __forceinline__ void store_state(const KArgs &A, long long p, const PState<DIM> &q) {
-    using F = PF<DIM>;
-#pragma unroll
-    for (int d = 0; d < DIM; d++) {
-        A.P[(F::X + d) * A.np + p] = q.x[d];
-        A.P[(F::V + d) * A.np + p] = q.v[d];
-    }
-#pragma unroll
-    for (int c = 0; c < F::NTS; c++) A.P[(F::T + c) * A.np + p] = q.T6[c];
-    A.P[F::VOL * A.np + p] = q.V;
-}
-
 // the per-point P2G payload: m, m v and V T (explicit_usl.cpp:132-161)
-template <int DIM> struct P2GVals {
-    double m;
-    double mv[DIM];
-    double VT[DIM][DIM];
+template <int DIM, class R> struct P2GVals {
+    R m;
+    R mv[DIM];
+    R VT[DIM][DIM];
 };
 
-template <int DIM>
-__device__ __forceinline__ void p2g_payload(const PState<DIM> &q, P2GVals<DIM> &pv) {
+template <int DIM, class R, class St>
+__device__ __forceinline__ void p2g_payload(const St &q, P2GVals<DIM, R> &pv) {
     pv.m = q.m;
 #pragma unroll
     for (int d = 0; d < DIM; d++) pv.mv[d] = q.v[d] * q.m;
-    const double Tf[3][3] = {{q.T6[0], q.T6[3], q.T6[4]}, {q.T6[3], q.T6[1], q.T6[5]}, {q.T6[4], q.T6[5], q.T6[2]}};
+    const R Tf[3][3] = {{q.T6[0], q.T6[3], q.T6[4]}, {q.T6[3], q.T6[1], q.T6[5]}, {q.T6[4], q.T6[5], q.T6[2]}};
 #pragma unroll
     for (int r = 0; r < DIM; r++)
 #pragma unroll
@@ -123,42 +91,42 @@ __device__ __forceinline__ void p2g_payload(const PState<DIM> &q, P2GVals<DIM> &
 }
 
 // node values of one P2G tuple: m, p[DIM], f[DIM] (stress divergence only), n[DIM]
-template <int DIM>
-__host__ __device__ __forceinline__ void p2g_node_values(const P2GVals<DIM> &q, double S, const double *G,
-                                                          double *out, bool b1) {
+template <int DIM, class R>
+__host__ __device__ __forceinline__ void p2g_node_values(const P2GVals<DIM, R> &q, R S, const R *G, R *out, bool b1) {
     out[0] = S * q.m;
 #pragma unroll
     for (int d = 0; d < DIM; d++) out[1 + d] = S * q.mv[d];
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        double f = 0.0;
+        R f = 0;
 #pragma unroll
         for (int c = 0; c < DIM; c++) f = fma(-q.VT[d][c], G[c], f);
         out[1 + DIM + d] = f;
     }
 #pragma unroll
-    for (int d = 0; d < DIM; d++) out[1 + 2 * DIM + d] = b1 ? G[d] * q.m : 0.0;
+    for (int d = 0; d < DIM; d++) out[1 + 2 * DIM + d] = b1 ? G[d] * q.m : (R)0;
 }
 
 // Visit the 3^DIM window slot by slot with the loop kept rolled (weights picked by
 // selects); used on the rare global-memory path so its loads do not set the kernel's
 // register budget. fn(off, S, G) with off the slot offsets from w.base.
-__host__ __device__ __forceinline__ double pick3(const double *a, int i) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
+template <class R>
+__host__ __device__ __forceinline__ R pick3(const R *a, int i) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
 
-template <int DIM, class Fn>
-__host__ __device__ __forceinline__ void for_each_sep(const Win<DIM> &w, double ext, Fn &&fn) {
-    const double gscale = 1.0 / ((double)(1 << DIM) * 0.5 * ext);
-    const double sscale = 1.0 / (double)(1 << DIM);
+template <int DIM, class R, class Fn>
+__host__ __device__ __forceinline__ void for_each_sep(const Win<DIM, R> &w, R ext, Fn &&fn) {
+    const R gscale = (R)1 / ((R)(1 << DIM) * (R)0.5 * ext);
+    const R sscale = (R)1 / (R)(1 << DIM);
     constexpr int NS = DIM == 3 ? 27 : 9;
 #pragma unroll 1
     for (int k = 0; k < NS; k++) {
         const int c0 = k % 3, c1 = (k / 3) % 3, c2 = k / 9;
-        const double sx = pick3(w.sw[0], c0), dx = pick3(w.dw[0], c0);
-        const double sy = pick3(w.sw[1], c1), dy = pick3(w.dw[1], c1);
-        const double sz = DIM == 3 ? pick3(w.sw[DIM - 1], c2) : 1.0;
-        const double dz = DIM == 3 ? pick3(w.dw[DIM - 1], c2) : 0.0;
-        const double S = sx * (sy * sz) * sscale;
-        double G[DIM];
+        const R sx = pick3(w.sw[0], c0), dx = pick3(w.dw[0], c0);
+        const R sy = pick3(w.sw[1], c1), dy = pick3(w.dw[1], c1);
+        const R sz = DIM == 3 ? pick3(w.sw[DIM - 1], c2) : (R)1;
+        const R dz = DIM == 3 ? pick3(w.dw[DIM - 1], c2) : (R)0;
+        const R S = sx * (sy * sz) * sscale;
+        R G[DIM];
         G[0] = dx * (sy * sz) * gscale;
         G[1] = sx * (dy * sz) * gscale;
         if (DIM == 3) G[DIM - 1] = sx * (sy * dz) * gscale;
@@ -172,16 +140,16 @@ __host__ __device__ __forceinline__ void for_each_sep(const Win<DIM> &w, double 
 
 // Row coefficients of the separable window: along a row (fixed c1, c2) the x-slot a has
 //   S = sw0[a] * Su,  G0 = dw0[a] * gx,  G1 = sw0[a] * gy,  G2 = sw0[a] * gz
-template <int DIM> struct RowW {
-    double Su, gx, gy, gz;
+template <class R> struct RowW {
+    R Su, gx, gy, gz;
 };
 
-template <int DIM>
-__device__ __forceinline__ RowW<DIM> row_weights(const Win<DIM> &w, int c1, int c2, double sscale, double gscale) {
-    const double sz = DIM == 3 ? w.sw[DIM - 1][c2] : 1.0;
-    const double dz = DIM == 3 ? w.dw[DIM - 1][c2] : 0.0;
-    const double syz = w.sw[1][c1] * sz;
-    RowW<DIM> r;
+template <int DIM, class R>
+__device__ __forceinline__ RowW<R> row_weights(const Win<DIM, R> &w, int c1, int c2, R sscale, R gscale) {
+    const R sz = DIM == 3 ? w.sw[DIM - 1][c2] : (R)1;
+    const R dz = DIM == 3 ? w.dw[DIM - 1][c2] : (R)0;
+    const R syz = w.sw[1][c1] * sz;
+    RowW<R> r;
     r.Su = syz * sscale;
     r.gx = syz * gscale;
     r.gy = (w.dw[1][c1] * sz) * gscale;
@@ -189,8 +157,8 @@ __device__ __forceinline__ RowW<DIM> row_weights(const Win<DIM> &w, int c1, int 
     return r;
 }
 
-template <int DIM>
-__device__ __forceinline__ bool window_in_patch(const Win<DIM> &w, const int *org) {
+template <int DIM, class R>
+__device__ __forceinline__ bool window_in_patch(const Win<DIM, R> &w, const int *org) {
     constexpr int P = Cfg<DIM>::P;
     bool fits = true;
 #pragma unroll
@@ -208,9 +176,9 @@ __device__ __forceinline__ bool node_in_patch(const int *nijk, const int *org) {
 }
 
 // G2P accumulation of one node: u += S v dt, dv += S dv, L += v (x) G
-template <int DIM>
-__device__ __forceinline__ void g2p_acc(const double *vn, const double *dvn, double S, const double *G, double dt,
-                                        double *up, double *dvp, double (*Lp)[DIM]) {
+template <int DIM, class R>
+__device__ __forceinline__ void g2p_acc(const R *vn, const R *dvn, R S, const R *G, R dt, R *up, R *dvp,
+                                        R (*Lp)[DIM]) {
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
         up[d] = fma(S, vn[d] * dt, up[d]);
@@ -222,38 +190,16 @@ __device__ __forceinline__ void g2p_acc(const double *vn, const double *dvn, dou
         for (int c = 0; c < DIM; c++) Lp[r][c] = fma(vn[r], G[c], Lp[r][c]);
 }
 
-// node values (v, dv) from the global grid
-template <int DIM>
+// node values (v, dv) from the global grid output
+template <int DIM, class PT>
 __device__ __forceinline__ void fetch_node(const KArgs &A, const SimDev &s, long long gbase, const int *nijk,
-                                           double *vn, double *dvn) {
+                                           typename PT::R *vn, typename PT::R *dvn) {
     using G = GF<DIM>;
     const long long gi = gbase + node_addr<DIM>(s, nijk);
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        vn[d] = A.out[(G::OV + d) * A.nt + gi];
-        dvn[d] = A.out[(G::ODV + d) * A.nt + gi];
-    }
-}
-
-// movePoints / calculateStrainRate / updateDensity / updateStress for one point
-template <int DIM>
-__device__ __forceinline__ void advance_point(const BodyDev &B, double dt, PState<DIM> &q, const double *up,
-                                              const double *dvp, const double (*Lp)[DIM]) {
-#pragma unroll
-    for (int d = 0; d < DIM; d++) {
-        q.x[d] = q.x[d] + up[d];   // x += S^T u     (explicit_usl.cpp:279)
-        q.v[d] = q.v[d] + dvp[d];  // x_t += S^T dv  (explicit_usl.cpp:292, FLIP)
-    }
-    double tr = Lp[0][0] + Lp[1][1];
-    if (DIM == 3) tr = tr + Lp[DIM - 1][DIM - 1];
-    q.V = q.V * exp(dt * tr);  // explicit_usl.cpp:338
-    if (B.has_material) {
-        double L3[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll
-        for (int r = 0; r < DIM; r++)
-#pragma unroll
-            for (int c = 0; c < DIM; c++) L3[r][c] = Lp[r][c];
-        isolin_update<double>(B.twoG, B.lambda, dt, L3, q.T6);
+        vn[d] = PT::fetch(A, (G::OV + d) * A.nt + gi);
+        dvn[d] = PT::fetch(A, (G::ODV + d) * A.nt + gi);
     }
 }
 
@@ -261,19 +207,24 @@ __device__ __forceinline__ void advance_point(const BodyDev &B, double dt, PStat
 //   DO_G2P: gather the grid of the current step (LDS patch), update the point
 //   DO_P2G: map the (updated) point and scatter it into the next step's accumulators
 // NRM: accumulate contact normals gradS*m for bodies that are body 1 of a contact.
-// LDS planes are [field][padded node index] (Cfg::PSTR doubles per plane).
-template <int DIM, bool DO_G2P, bool DO_P2G, bool NRM>
+// PT: particle-state form (Prec64 / Prec32). LDS planes are [field][padded node index]
+// (Cfg::PSTR elements per plane); the accumulator patch is FP64 in both forms, the
+// gathered grid patch has the grid output's type.
+template <int DIM, bool DO_G2P, bool DO_P2G, bool NRM, class PT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_particles(KArgs A, int want_L) {
+    using R = typename PT::R;
+    using GO = typename PT::GO;
+    using St = typename PT::template State<DIM>;
     constexpr int PATCH = Cfg<DIM>::PATCH, P = Cfg<DIM>::P, PSTR = Cfg<DIM>::PSTR, ZS = Cfg<DIM>::ZS;
     constexpr int NACC = NRM ? GF<DIM>::NACC : 1 + 2 * DIM;
     constexpr int NOUT = GF<DIM>::NOUT;
     constexpr int NZ = DIM == 3 ? 3 : 1;
     __shared__ double lacc[DO_P2G ? NACC * PSTR : 1];
-    __shared__ double gpat[DO_G2P ? NOUT * PSTR : 1];
+    __shared__ GO gpat[DO_G2P ? NOUT * PSTR : 1];
     __shared__ int bb[2 * DIM];
     const int tid = threadIdx.x;
     const int nent = *A.npblk;
-    const double sscale = 1.0 / (double)(1 << DIM);
+    const R sscale = (R)1 / (R)(1 << DIM);
 #pragma unroll 1
     for (int e = blockIdx.x; e < nent; e += gridDim.x) {
         const PBlk ent = A.plist[e];
@@ -284,6 +235,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const bool b1 = NRM && B.is_b1 != 0;
         const long long gbase = B.grid_base;
         const double dt = s.dt;
+        const R dtr = (R)dt;
         int org[DIM];
         patch_origin<DIM>(s, uniform(ent.key) - B.pblk_off, org);
         if (DO_P2G) {
@@ -306,8 +258,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     r /= P;
                     in = in && nijk[d] >= 0 && nijk[d] <= s.N[d];
                 }
-                double v = 0.0;
-                if (in) v = A.out[f * A.nt + gbase + node_addr<DIM>(s, nijk)];
+                GO v = 0;
+                if (in) v = PT::fetch(A, f * A.nt + gbase + node_addr<DIM>(s, nijk));
                 gpat[f * PSTR + patch_index<DIM>(nijk, org)] = v;
             }
         }
@@ -316,48 +268,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int k = tid; k < count; k += blockDim.x) {
             const long long p = start + k;
             if (!A.act[p]) continue;
-            PState<DIM> q;
-            load_state<DIM>(A, p, q);
+            St q;
+            PT::template load<DIM>(A, p, q, DO_P2G);
             if (DO_G2P) {
-                Win<DIM> w;
+                Win<DIM, R> w;
                 bool cpdi;
-                const int mode = point_window<DIM>(s, q.x, q.ext, w, cpdi);
+                const int mode = PT::template window<DIM>(s, q, w, cpdi);
                 if (mode == MODE_LEFT) {  // not reachable after this step's map; kept for safety
                     A.act[p] = 0;
                     continue;
                 }
-                double up[DIM], dvp[DIM], Lp[DIM][DIM];
+                R up[DIM], dvp[DIM], Lp[DIM][DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; d++) {
-                    up[d] = dvp[d] = 0.0;
+                    up[d] = dvp[d] = 0;
 #pragma unroll
-                    for (int c = 0; c < DIM; c++) Lp[d][c] = 0.0;
+                    for (int c = 0; c < DIM; c++) Lp[d][c] = 0;
                 }
                 if (mode == MODE_SEP && window_in_patch<DIM>(w, org)) {
                     // row-factored gather: per row, U = sum sw0 v, DV = sum sw0 dv, LX = sum dw0 v
-                    const double gscale = 1.0 / ((double)(1 << DIM) * 0.5 * q.ext);
+                    const R gscale = (R)1 / ((R)(1 << DIM) * (R)0.5 * q.ext);
                     const int i0 = patch_index<DIM>(w.base, org);
 #pragma unroll
                     for (int c2 = 0; c2 < NZ; c2++) {
 #pragma unroll
                         for (int c1 = 0; c1 < 3; c1++) {
-                            const RowW<DIM> rw = row_weights<DIM>(w, c1, c2, sscale, gscale);
-                            double U[DIM], DV[DIM], LX[DIM];
+                            const RowW<R> rw = row_weights<DIM, R>(w, c1, c2, sscale, gscale);
+                            R U[DIM], DV[DIM], LX[DIM];
 #pragma unroll
-                            for (int d = 0; d < DIM; d++) U[d] = DV[d] = LX[d] = 0.0;
+                            for (int d = 0; d < DIM; d++) U[d] = DV[d] = LX[d] = 0;
 #pragma unroll
                             for (int c0 = 0; c0 < 3; c0++) {
                                 const int i = i0 + c0 + P * c1 + ZS * c2;
 #pragma unroll
                                 for (int d = 0; d < DIM; d++) {
-                                    const double vn = gpat[d * PSTR + i];
-                                    const double dvn = gpat[(DIM + d) * PSTR + i];
+                                    const R vn = gpat[d * PSTR + i];
+                                    const R dvn = gpat[(DIM + d) * PSTR + i];
                                     U[d] = fma(w.sw[0][c0], vn, U[d]);
                                     DV[d] = fma(w.sw[0][c0], dvn, DV[d]);
                                     LX[d] = fma(w.dw[0][c0], vn, LX[d]);
                                 }
                             }
-                            const double sdt = rw.Su * dt;
+                            const R sdt = rw.Su * dtr;
 #pragma unroll
                             for (int d = 0; d < DIM; d++) {
                                 up[d] = fma(sdt, U[d], up[d]);
@@ -371,17 +323,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                         }
                     }
                 } else if (mode == MODE_SEP) {
-                    for_each_sep<DIM>(w, q.ext, [&](const int *off, double S, const double *G) {
+                    for_each_sep<DIM, R>(w, q.ext, [&](const int *off, R S, const R *G) {
                         int nijk[DIM];
 #pragma unroll
                         for (int d = 0; d < DIM; d++) nijk[d] = w.base[d] + off[d];
-                        double vn[DIM], dvn[DIM];
-                        fetch_node<DIM>(A, s, gbase, nijk, vn, dvn);
-                        g2p_acc<DIM>(vn, dvn, S, G, dt, up, dvp, Lp);
+                        R vn[DIM], dvn[DIM];
+                        fetch_node<DIM, PT>(A, s, gbase, nijk, vn, dvn);
+                        g2p_acc<DIM, R>(vn, dvn, S, G, dtr, up, dvp, Lp);
                     });
                 } else if (mode == MODE_TUP) {
-                    for_each_tuple<DIM>(s, q.x, q.ext, cpdi, [&](const int *nijk, double S, const double *G) {
-                        double vn[DIM], dvn[DIM];
+                    PT::template tuples<DIM>(s, q, cpdi, [&](const int *nijk, R S, const R *G) {
+                        R vn[DIM], dvn[DIM];
                         if (node_in_patch<DIM>(nijk, org)) {
                             const int i = patch_index<DIM>(nijk, org);
 #pragma unroll
@@ -390,13 +342,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                                 dvn[d] = gpat[(DIM + d) * PSTR + i];
                             }
                         } else {
-                            fetch_node<DIM>(A, s, gbase, nijk, vn, dvn);
+                            fetch_node<DIM, PT>(A, s, gbase, nijk, vn, dvn);
                         }
-                        g2p_acc<DIM>(vn, dvn, S, G, dt, up, dvp, Lp);
+                        g2p_acc<DIM, R>(vn, dvn, S, G, dtr, up, dvp, Lp);
                     });
                 }
-                advance_point<DIM>(B, dt, q, up, dvp, Lp);
-                store_state<DIM>(A, p, q);
+                PT::template advance<DIM>(B, s, dt, q, up, dvp, Lp);
+                PT::template store<DIM>(A, p, q);
                 if (want_L && A.Lout) {
                     double L9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -408,24 +360,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 }
             }
             if (DO_P2G) {
-                Win<DIM> w;
+                Win<DIM, R> w;
                 bool cpdi;
-                const int mode = point_window<DIM>(s, q.x, q.ext, w, cpdi);
+                const int mode = PT::template window<DIM>(s, q, w, cpdi);
                 if (mode == MODE_LEFT) {  // leaves the domain: deactivated (default_points.cpp:217-219)
                     A.act[p] = 0;
                     continue;
                 }
                 if (mode == MODE_NONE) continue;
-                P2GVals<DIM> pv;
-                p2g_payload<DIM>(q, pv);
-                auto scatter_global = [&](const int *nijk, double S, const double *G) {
-                    double vals[GF<DIM>::NACC];
-                    p2g_node_values<DIM>(pv, S, G, vals, b1);
+                P2GVals<DIM, R> pv;
+                p2g_payload<DIM, R>(q, pv);
+                auto scatter_global = [&](const int *nijk, R S, const R *G) {
+                    R vals[GF<DIM>::NACC];
+                    p2g_node_values<DIM, R>(pv, S, G, vals, b1);
                     const long long gi = gbase + node_addr<DIM>(s, nijk);
 #pragma unroll
                     for (int f = 0; f < NACC; f++) {
                         if (f >= 1 + 2 * DIM && !b1) continue;
-                        unsafeAtomicAdd(&A.acc[f * A.nt + gi], vals[f]);
+                        unsafeAtomicAdd(&A.acc[f * A.nt + gi], (double)vals[f]);
                     }
                 };
                 if (mode == MODE_SEP && window_in_patch<DIM>(w, org)) {
@@ -435,38 +387,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                         atomicMax(&bb[DIM + d], w.base[d] + 2);
                     }
                     // row-factored scatter: along a row every field is a * sw0 + b * dw0
-                    const double gscale = 1.0 / ((double)(1 << DIM) * 0.5 * q.ext);
+                    const R gscale = (R)1 / ((R)(1 << DIM) * (R)0.5 * q.ext);
                     const int i0 = patch_index<DIM>(w.base, org);
 #pragma unroll
                     for (int c2 = 0; c2 < NZ; c2++) {
 #pragma unroll
                         for (int c1 = 0; c1 < 3; c1++) {
-                            const RowW<DIM> rw = row_weights<DIM>(w, c1, c2, sscale, gscale);
-                            const double am = pv.m * rw.Su;
-                            double ap[DIM], af0[DIM], af1[DIM];
+                            const RowW<R> rw = row_weights<DIM, R>(w, c1, c2, sscale, gscale);
+                            const R am = pv.m * rw.Su;
+                            R ap[DIM], af0[DIM], af1[DIM];
 #pragma unroll
                             for (int d = 0; d < DIM; d++) {
                                 ap[d] = pv.mv[d] * rw.Su;
                                 af0[d] = -(pv.VT[d][0] * rw.gx);
-                                double t = pv.VT[d][1] * rw.gy;
+                                R t = pv.VT[d][1] * rw.gy;
                                 if (DIM == 3) t = fma(pv.VT[d][DIM - 1], rw.gz, t);
                                 af1[d] = -t;
                             }
-                            const double an0 = pv.m * rw.gx, an1 = pv.m * rw.gy, an2 = pv.m * rw.gz;
+                            const R an0 = pv.m * rw.gx, an1 = pv.m * rw.gy, an2 = pv.m * rw.gz;
 #pragma unroll
                             for (int c0 = 0; c0 < 3; c0++) {
                                 const int i = i0 + c0 + P * c1 + ZS * c2;
-                                const double sx = w.sw[0][c0], dx = w.dw[0][c0];
-                                atomicAdd(&lacc[GF<DIM>::M * PSTR + i], am * sx);
+                                const R sx = w.sw[0][c0], dx = w.dw[0][c0];
+                                atomicAdd(&lacc[GF<DIM>::M * PSTR + i], (double)(am * sx));
 #pragma unroll
                                 for (int d = 0; d < DIM; d++) {
-                                    atomicAdd(&lacc[(GF<DIM>::P + d) * PSTR + i], ap[d] * sx);
-                                    atomicAdd(&lacc[(GF<DIM>::F + d) * PSTR + i], fma(af0[d], dx, af1[d] * sx));
+                                    atomicAdd(&lacc[(GF<DIM>::P + d) * PSTR + i], (double)(ap[d] * sx));
+                                    atomicAdd(&lacc[(GF<DIM>::F + d) * PSTR + i], (double)fma(af0[d], dx, af1[d] * sx));
                                 }
                                 if (NRM && b1) {
-                                    atomicAdd(&lacc[(GF<DIM>::NRM + 0) * PSTR + i], an0 * dx);
-                                    atomicAdd(&lacc[(GF<DIM>::NRM + 1) * PSTR + i], an1 * sx);
-                                    if (DIM == 3) atomicAdd(&lacc[(GF<DIM>::NRM + DIM - 1) * PSTR + i], an2 * sx);
+                                    atomicAdd(&lacc[(GF<DIM>::NRM + 0) * PSTR + i], (double)(an0 * dx));
+                                    atomicAdd(&lacc[(GF<DIM>::NRM + 1) * PSTR + i], (double)(an1 * sx));
+                                    if (DIM == 3)
+                                        atomicAdd(&lacc[(GF<DIM>::NRM + DIM - 1) * PSTR + i], (double)(an2 * sx));
                                 }
                             }
                         }
@@ -479,26 +432,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                         hi[d] = w.base[d] + 2;
                     }
                     mark_range<DIM>(A, s, lo, hi);
-                    for_each_sep<DIM>(w, q.ext, [&](const int *off, double S, const double *G) {
+                    for_each_sep<DIM, R>(w, q.ext, [&](const int *off, R S, const R *G) {
                         int nijk[DIM];
 #pragma unroll
                         for (int d = 0; d < DIM; d++) nijk[d] = w.base[d] + off[d];
                         scatter_global(nijk, S, G);
                     });
                 } else {  // MODE_TUP
-                    for_each_tuple<DIM>(s, q.x, q.ext, cpdi, [&](const int *nijk, double S, const double *G) {
+                    PT::template tuples<DIM>(s, q, cpdi, [&](const int *nijk, R S, const R *G) {
                         if (node_in_patch<DIM>(nijk, org)) {
 #pragma unroll
                             for (int d = 0; d < DIM; d++) {
                                 atomicMin(&bb[d], nijk[d]);
                                 atomicMax(&bb[DIM + d], nijk[d]);
                             }
-                            double vals[GF<DIM>::NACC];
-                            p2g_node_values<DIM>(pv, S, G, vals, b1);
+                            R vals[GF<DIM>::NACC];
+                            p2g_node_values<DIM, R>(pv, S, G, vals, b1);
                             const int i = patch_index<DIM>(nijk, org);
                             for (int f = 0; f < NACC; f++) {
                                 if (f >= 1 + 2 * DIM && !b1) continue;
-                                atomicAdd(&lacc[f * PSTR + i], vals[f]);
+                                atomicAdd(&lacc[f * PSTR + i], (double)vals[f]);
                             }
                         } else {
                             mark_range<DIM>(A, s, nijk, nijk);
@@ -549,8 +502,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 // ---------------------------------------------------------------------------------------
 // grid update: body force, contact, boundary, moveGrid. One wavefront per node block.
-template <int DIM>
+template <int DIM, class GO>
 __global__ __launch_bounds__(256) void k_grid(KArgs A) {
+    GO *outp = nullptr;
+    if constexpr (sizeof(GO) == 8) outp = reinterpret_cast<GO *>(A.out);
+    else outp = reinterpret_cast<GO *>(A.out32);
     using G = GF<DIM>;
     constexpr int NB = Cfg<DIM>::NB, L = Cfg<DIM>::LNB;
     const int par = A.par;
@@ -714,8 +670,8 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
 #pragma unroll
                 for (int d = 0; d < DIM; d++) {
                     const double pn = p[b][d] + f[b][d] * dt;
-                    A.out[(G::OV + d) * A.nt + gi] = has ? pn * rm : 0.0;
-                    A.out[(G::ODV + d) * A.nt + gi] = has ? (f[b][d] * dt) * rm : 0.0;
+                    outp[(G::OV + d) * A.nt + gi] = (GO)(has ? pn * rm : 0.0);
+                    outp[(G::ODV + d) * A.nt + gi] = (GO)(has ? (f[b][d] * dt) * rm : 0.0);
                 }
             }
         }
@@ -731,9 +687,8 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
 
 // ---------------------------------------------------------------------------------------
 // connectivity of the current state (DefaultPoints::generateMap's element lookups)
-template <int DIM>
+template <int DIM, class PT>
 __global__ void k_connectivity(KArgs A, int *elem_out, uint8_t *code_out) {
-    using F = PF<DIM>;
     constexpr int NC = 1 << DIM;
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= A.np) return;
@@ -751,10 +706,8 @@ __global__ void k_connectivity(KArgs A, int *elem_out, uint8_t *code_out) {
     }
     const SimDev &s = A.sims[A.bodies[bi].sim];
     for (int c = 0; c < NC; c++) elem_out[(long long)o * NC + c] = -1;
-    double x[DIM];
-#pragma unroll
-    for (int d = 0; d < DIM; d++) x[d] = A.P[(F::X + d) * A.np + p];
-    const double ext = A.P[F::EXT * A.np + p];
+    double x[DIM], ext;
+    PT::template position<DIM>(A, s, p, x, ext);
     if (!A.act[p] || !in_domain<DIM>(s, x)) {
         code_out[o] = 2;
         return;

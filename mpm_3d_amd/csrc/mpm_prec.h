@@ -1,0 +1,388 @@
+// mpm_prec.h -- the two particle-state forms behind one kernel skeleton.
+//
+//   Prec64  plain form: FP64 state x, v, T, V, m, extent; the element search and the GIMP
+//           corner arithmetic round exactly as the reference's (cartesian.cpp:219-253,
+//           default_points.cpp:196-330), so connectivity is bit-identical.
+//   Prec32  precision form: a packed cell index plus a 32-bit fixed-point offset inside
+//           the cell (x = h * (cell + q 2^-32)), FP32 v, T, m, extent and the volume as
+//           V0 exp(lnJ); shape functions, G2P and the IsotropicLinearElasticity update in
+//           FP32. Nodal mass,
+//           momentum, force and contact normals still accumulate in FP64 (LDS and global
+//           atomics), and the grid update (p/m division, ContactHuang) stays FP64; only
+//           its output v, dv is stored as FP32 for the gather.
+//
+// Each form provides: State, load/store, window (separable GIMP weights), tuples (per-tuple
+// weights for CPDI-off points and the shapes the separable form does not cover), advance
+// (movePoints .. updateStress), fetch (one grid-output value), the sort key inputs and the
+// FP64 position used by the connectivity query.
+#pragma once
+
+#include "mpm_common.h"
+
+namespace mpm {
+
+struct Prec64 {
+    using R = double;   // particle arithmetic
+    using GO = double;  // grid output element
+    static constexpr bool FP32 = false;
+    template <int DIM> struct State {
+        double x[DIM], v[DIM], T6[6], V, m, ext;
+    };
+
+    template <int DIM>
+    __device__ __forceinline__ static void load(const KArgs &A, long long p, State<DIM> &q, bool) {
+        using F = PF<DIM>;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            q.x[d] = A.P[(F::X + d) * A.np + p];
+            q.v[d] = A.P[(F::V + d) * A.np + p];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; c++) q.T6[c] = 0.0;
+#pragma unroll
+        for (int c = 0; c < F::NTS; c++) q.T6[c] = A.P[(F::T + c) * A.np + p];
+        q.V = A.P[F::VOL * A.np + p];
+        q.m = A.P[F::MASS * A.np + p];
+        q.ext = A.P[F::EXT * A.np + p];
+    }
+
+    template <int DIM>
+    __device__ __forceinline__ static void store(const KArgs &A, long long p, const State<DIM> &q) {
+        using F = PF<DIM>;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            A.P[(F::X + d) * A.np + p] = q.x[d];
+            A.P[(F::V + d) * A.np + p] = q.v[d];
+        }
+#pragma unroll
+        for (int c = 0; c < F::NTS; c++) A.P[(F::T + c) * A.np + p] = q.T6[c];
+        A.P[F::VOL * A.np + p] = q.V;
+    }
+
+    template <int DIM>
+    __device__ __forceinline__ static int window(const SimDev &s, const State<DIM> &q, Win<DIM, double> &w,
+                                                 bool &cpdi) {
+        return point_window<DIM>(s, q.x, q.ext, w, cpdi);
+    }
+
+    template <int DIM, class Fn>
+    __device__ __forceinline__ static void tuples(const SimDev &s, const State<DIM> &q, bool cpdi, Fn &&fn) {
+        for_each_tuple<DIM>(s, q.x, q.ext, cpdi, fn);
+    }
+
+    __device__ __forceinline__ static double fetch(const KArgs &A, long long i) { return A.out[i]; }
+
+    // movePoints / calculateStrainRate / updateDensity / updateStress (explicit_usl.cpp:266-355)
+    template <int DIM>
+    __device__ __forceinline__ static void advance(const BodyDev &B, const SimDev &s, double dt, State<DIM> &q,
+                                                   const double *up, const double *dvp, const double (*Lp)[DIM]) {
+        (void)s;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            q.x[d] = q.x[d] + up[d];   // x += S^T u     (explicit_usl.cpp:279)
+            q.v[d] = q.v[d] + dvp[d];  // x_t += S^T dv  (explicit_usl.cpp:292, FLIP)
+        }
+        double tr = Lp[0][0] + Lp[1][1];
+        if (DIM == 3) tr = tr + Lp[DIM - 1][DIM - 1];
+        q.V = q.V * exp(dt * tr);  // explicit_usl.cpp:338
+        if (B.has_material) {
+            double L3[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+            for (int r = 0; r < DIM; r++)
+#pragma unroll
+                for (int c = 0; c < DIM; c++) L3[r][c] = Lp[r][c];
+            isolin_update<double>(B.twoG, B.lambda, dt, L3, q.T6);
+        }
+    }
+
+    // sort key inputs: dual cell (shifted to >= 0) and octant inside it; false if the
+    // point is outside [0, L]
+    template <int DIM>
+    __device__ __forceinline__ static bool dual_cell(const KArgs &A, const SimDev &s, long long p, int *dual, int &oct) {
+        double x[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) x[d] = A.P[(PF<DIM>::X + d) * A.np + p];
+        if (!in_domain<DIM>(s, x)) return false;
+        oct = 0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const double t = x[d] * s.invh[d] - 0.5;
+            const double ft = floor(t);
+            int c = (int)ft + 1;
+            dual[d] = c < 0 ? 0 : (c > s.N[d] ? s.N[d] : c);
+            oct |= (t - ft >= 0.5 ? 1 : 0) << d;
+        }
+        return true;
+    }
+
+    template <int DIM>
+    __device__ __forceinline__ static void position(const KArgs &A, const SimDev &s, long long p, double *x,
+                                                    double &ext) {
+        (void)s;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) x[d] = A.P[(PF<DIM>::X + d) * A.np + p];
+        ext = A.P[PF<DIM>::EXT * A.np + p];
+    }
+};
+
+struct Prec32 {
+    using R = float;
+    using GO = float;
+    static constexpr bool FP32 = true;
+    template <int DIM> struct State {
+        int c[DIM];       // cell
+        uint32_t q[DIM];  // offset inside the cell, 32-bit fixed point (x = h (c + q 2^-32))
+        float r[DIM];     // q 2^-32 rounded to FP32, for the shape functions
+        float v[DIM], T6[6], lnJ, V, m, ext;  // V = V0 exp(lnJ), loaded for P2G only
+    };
+
+    template <int DIM>
+    __device__ __forceinline__ static void load(const KArgs &A, long long p, State<DIM> &q, bool need_vol) {
+        using F = PW<DIM>;
+        const float *Pf = reinterpret_cast<const float *>(A.Pw);
+        unpack_cell<DIM>(A.Pw[F::CELL * A.np + p], q.c);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            q.q[d] = A.Pw[(F::Q + d) * A.np + p];
+            q.r[d] = (float)q.q[d] * 0x1p-32f;
+            q.v[d] = Pf[(F::V + d) * A.np + p];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; c++) q.T6[c] = 0.0f;
+#pragma unroll
+        for (int c = 0; c < F::NTS; c++) q.T6[c] = Pf[(F::T + c) * A.np + p];
+        q.lnJ = Pf[F::LNJ * A.np + p];
+        q.V = need_vol ? Pf[F::V0 * A.np + p] * expf(q.lnJ) : 0.0f;
+        q.m = Pf[F::MASS * A.np + p];
+        q.ext = Pf[F::EXT * A.np + p];
+    }
+
+    template <int DIM>
+    __device__ __forceinline__ static void store(const KArgs &A, long long p, const State<DIM> &q) {
+        using F = PW<DIM>;
+        float *Pf = reinterpret_cast<float *>(A.Pw);
+        A.Pw[F::CELL * A.np + p] = pack_cell<DIM>(q.c);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            A.Pw[(F::Q + d) * A.np + p] = q.q[d];
+            Pf[(F::V + d) * A.np + p] = q.v[d];
+        }
+#pragma unroll
+        for (int c = 0; c < F::NTS; c++) Pf[(F::T + c) * A.np + p] = q.T6[c];
+        Pf[F::LNJ * A.np + p] = q.lnJ;
+    }
+
+    // x = h (c + r) is inside [0, L] in dimension d: element E = c + floor(r) with
+    // 0 <= E < N, or exactly on the last node (E == N, frac 0). NaN is outside.
+    __device__ __forceinline__ static bool inside(int c, float r, int N) {
+        if (!(fabsf(r) < 8.0f)) return false;  // NaN, or far outside any cell of c
+        const float fl = floorf(r);
+        const int E = c + (int)fl;
+        return (E >= 0 && E < N) || (E == N && r == fl);
+    }
+
+    template <int DIM>
+    __device__ __forceinline__ static int window(const SimDev &s, const State<DIM> &q, Win<DIM, float> &w,
+                                                 bool &cpdi) {
+        cpdi = false;
+        bool in = true;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) in = in && inside(q.c[d], q.r[d], s.N[d]);
+        if (!in) return MODE_LEFT;
+        if (s.cpdi_spec != 1) return MODE_TUP;
+        float cm[DIM], cp[DIM];
+        bool cin = true;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const float a = 0.5f * q.ext * s.invhf[d];  // half extent in cells
+            cm[d] = q.r[d] - a;
+            cp[d] = q.r[d] + a;
+            cin = cin && inside(q.c[d], cm[d], s.N[d]) && inside(q.c[d], cp[d], s.N[d]);
+        }
+        if (!cin) return MODE_TUP;  // a corner leaves the domain: CPDI off
+        cpdi = true;
+        int mode = MODE_SEP;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const float flm = floorf(cm[d]), flp = floorf(cp[d]);
+            const int em = q.c[d] + (int)flm, ep = q.c[d] + (int)flp;
+            const float fm = cm[d] - flm, fp = cp[d] - flp;
+            const bool vm = em < s.N[d], vp = ep < s.N[d];  // a corner exactly at L has no element
+            if (!vm && !vp) mode = MODE_NONE;
+            const int lo = vm ? em : ep;
+            const int hi = vp ? ep : em;
+            if (hi - lo > 1) mode = mode == MODE_NONE ? MODE_NONE : MODE_TUP;
+            int b = q.c[d] + (int)floorf(q.r[d] - 0.5f);
+            b = b < hi - 1 ? hi - 1 : (b > lo ? lo : b);
+            w.base[d] = b;
+            float s0 = 0, s1 = 0, s2 = 0, d0 = 0, d1 = 0, d2 = 0;
+            if (vm) {
+                const float h0 = 1.0f - fm, h1 = fm;
+                const bool k0 = em == b;
+                s0 += k0 ? h0 : 0.0f;
+                s1 += k0 ? h1 : h0;
+                s2 += k0 ? 0.0f : h1;
+                d0 -= k0 ? h0 : 0.0f;
+                d1 -= k0 ? h1 : h0;
+                d2 -= k0 ? 0.0f : h1;
+            }
+            if (vp) {
+                const float h0 = 1.0f - fp, h1 = fp;
+                const bool k0 = ep == b;
+                s0 += k0 ? h0 : 0.0f;
+                s1 += k0 ? h1 : h0;
+                s2 += k0 ? 0.0f : h1;
+                d0 += k0 ? h0 : 0.0f;
+                d1 += k0 ? h1 : h0;
+                d2 += k0 ? 0.0f : h1;
+            }
+            w.sw[d][0] = s0; w.sw[d][1] = s1; w.sw[d][2] = s2;
+            w.dw[d][0] = d0; w.dw[d][1] = d1; w.dw[d][2] = d2;
+        }
+        return mode;
+    }
+
+    // per-tuple weights in cell-relative coordinates (same tuple set as the reference's
+    // COO construction: corners, or the point itself with CPDI off)
+    template <int DIM, class Fn>
+    __device__ __forceinline__ static void tuples(const SimDev &s, const State<DIM> &q, bool cpdi, Fn &&fn) {
+        constexpr int NC = 1 << DIM;
+        const int ncorner = cpdi ? NC : 1;
+        for (int c = 0; c < ncorner; c++) {
+            int E[DIM];
+            float f[DIM];
+            bool valid = true;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                const float A = ((c >> d) & 1) ? 1.0f : -1.0f;
+                const float rr = cpdi ? q.r[d] + 0.5f * q.ext * s.invhf[d] * A : q.r[d];
+                const float fl = floorf(rr);
+                E[d] = q.c[d] + (int)fl;
+                f[d] = rr - fl;
+                valid = valid && E[d] >= 0 && E[d] < s.N[d];
+            }
+            if (!valid) continue;
+            for (int n = 0; n < NC; n++) {
+                int nijk[DIM];
+                float rn[DIM];
+                float tmp = 1.0f;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    const int bit = (n >> d) & 1;
+                    nijk[d] = E[d] + bit;
+                    rn[d] = f[d] - (float)bit;  // (x - x_node) / h
+                    tmp *= (1.0f - fabsf(rn[d]));
+                }
+                float S, G[DIM];
+                if (cpdi) {
+                    S = tmp / (float)NC;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        const float A = ((c >> d) & 1) ? 1.0f : -1.0f;
+                        G[d] = tmp / ((float)NC * 0.5f * q.ext) * A;
+                    }
+                } else {
+                    S = tmp;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        // cartesian.cpp:325-333, including 0/0 = NaN on a node line
+                        float gi = -tmp;
+                        gi = gi / (1.0f - fabsf(rn[d]));
+                        gi = gi * rn[d];
+                        gi = gi / fabsf(rn[d]);
+                        gi = gi / s.hf[d];
+                        if (fabsf(rn[d]) >= 1.0f) gi = 0.0f;
+                        G[d] = gi;
+                    }
+                }
+                fn(nijk, S, G);
+            }
+        }
+    }
+
+    __device__ __forceinline__ static float fetch(const KArgs &A, long long i) { return A.out32[i]; }
+
+    // Positions advance in 32-bit fixed point inside the cell, so increments far below an
+    // FP32 ulp of the offset (slow points) still accumulate; the volume advances as
+    // lnJ += dt tr L, exact in the small increments where V *= exp(dt tr L) in FP32 is not.
+    template <int DIM>
+    __device__ __forceinline__ static void advance(const BodyDev &B, const SimDev &s, double dt, State<DIM> &q,
+                                                   const float *up, const float *dvp, const float (*Lp)[DIM]) {
+        const float dtf = (float)dt;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const float inc = up[d] * s.invhf[d];  // cells
+            if (fabsf(inc) < 4.0f) {
+                const long long t = (long long)q.q[d] + (long long)rintf(inc * 0x1p32f);
+                int c = q.c[d] + (int)(t >> 32);  // floor(t / 2^32)
+                uint32_t f = (uint32_t)t;
+                // left the domain by more than a cell in one step: keep it outside, within
+                // the packed range (cell -1, or half a cell past L)
+                if (c < -1) {
+                    c = -1;
+                    f = 0u;
+                } else if (c > s.N[d]) {
+                    c = s.N[d];
+                    f = 0x80000000u;
+                }
+                q.c[d] = c;
+                q.q[d] = f;
+            } else {  // NaN or a jump of 4+ cells: park the point outside the grid
+                q.c[d] = -1;
+                q.q[d] = 0u;
+            }
+            q.v[d] = q.v[d] + dvp[d];
+        }
+        float tr = Lp[0][0] + Lp[1][1];
+        if (DIM == 3) tr = tr + Lp[DIM - 1][DIM - 1];
+        q.lnJ = q.lnJ + dtf * tr;
+        if (B.has_material) {
+            float L3[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+            for (int r = 0; r < DIM; r++)
+#pragma unroll
+                for (int c = 0; c < DIM; c++) L3[r][c] = Lp[r][c];
+            isolin_update<float>(B.twoG, B.lambda, dt, L3, q.T6);
+        }
+    }
+
+    template <int DIM>
+    __device__ __forceinline__ static bool dual_cell(const KArgs &A, const SimDev &s, long long p, int *dual, int &oct) {
+        using F = PW<DIM>;
+        int c[DIM];
+        unpack_cell<DIM>(A.Pw[F::CELL * A.np + p], c);
+        uint32_t qq[DIM];
+        bool in = true;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            qq[d] = A.Pw[(F::Q + d) * A.np + p];
+            in = in && inside(c[d], (float)qq[d] * 0x1p-32f, s.N[d]);
+        }
+        if (!in) return false;
+        oct = 0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            // dual cell of x/h - 1/2 and the half of it the point is in
+            const bool upper = qq[d] >= 0x80000000u;
+            int dc = c[d] + (upper ? 1 : 0);
+            dual[d] = dc < 0 ? 0 : (dc > s.N[d] ? s.N[d] : dc);
+            oct |= (upper ? 0 : 1) << d;
+        }
+        return true;
+    }
+
+    template <int DIM>
+    __device__ __forceinline__ static void position(const KArgs &A, const SimDev &s, long long p, double *x,
+                                                    double &ext) {
+        using F = PW<DIM>;
+        const float *Pf = reinterpret_cast<const float *>(A.Pw);
+        int c[DIM];
+        unpack_cell<DIM>(A.Pw[F::CELL * A.np + p], c);
+#pragma unroll
+        for (int d = 0; d < DIM; d++)
+            x[d] = s.h[d] * ((double)c[d] + (double)A.Pw[(F::Q + d) * A.np + p] * 0x1p-32);
+        ext = Pf[F::EXT * A.np + p];
+    }
+};
+
+}  // namespace mpm

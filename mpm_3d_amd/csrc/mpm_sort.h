@@ -18,6 +18,7 @@
 #pragma once
 
 #include "mpm_common.h"
+#include "mpm_prec.h"
 
 namespace mpm {
 
@@ -41,10 +42,9 @@ __device__ __forceinline__ int body_of_block(const KArgs &A, long long key) {
     return lo;
 }
 
-template <int DIM>
+template <int DIM, class PT>
 __global__ __launch_bounds__(256) void k_bin_count(KArgs A, unsigned *count, int *pkey, int *prank,
                                                    long long inactive_key) {
-    using F = PF<DIM>;
     constexpr int PB = Cfg<DIM>::PB;
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) *const_cast<int *>(A.npblk) = 0;
@@ -54,21 +54,13 @@ __global__ __launch_bounds__(256) void k_bin_count(KArgs A, unsigned *count, int
         const int b = body_of_index(A, A.pid[p]);
         const BodyDev &B = A.bodies[b];
         const SimDev &s = A.sims[B.sim];
-        double x[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; d++) x[d] = A.P[(F::X + d) * A.np + p];
-        if (in_domain<DIM>(s, x)) {
+        int dual[DIM], oct = 0;
+        if (PT::template dual_cell<DIM>(A, s, p, dual, oct)) {
             long long lk = 0, stride = 1;
-            int oct = 0;
 #pragma unroll
             for (int d = 0; d < DIM; d++) {
-                const double t = x[d] * s.invh[d] - 0.5;
-                const double ft = floor(t);
-                int c = (int)ft + 1;  // dual cell, shifted to >= 0
-                c = c < 0 ? 0 : (c > s.N[d] ? s.N[d] : c);
-                lk += (long long)(c / PB) * stride;
+                lk += (long long)(dual[d] / PB) * stride;
                 stride *= s.PNB[d];
-                oct |= (t - ft >= 0.5 ? 1 : 0) << d;
             }
             key = (B.pblk_off + lk) * (1 << DIM) + oct;
         } else {
@@ -159,8 +151,8 @@ __global__ __launch_bounds__(256) void k_clear(unsigned *count, long long n) {
     if (k < n) count[k] = 0u;
 }
 
-template <int NF>
-__global__ __launch_bounds__(256) void k_bin_scatter(long long np, const double *P, double *P2, const int8_t *act,
+template <class E, int NF>
+__global__ __launch_bounds__(256) void k_bin_scatter(long long np, const E *P, E *P2, const int8_t *act,
                                                      int8_t *act2, const int *pid, int *pid2, const int *pkey,
                                                      const int *prank, const unsigned *off) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -173,8 +165,9 @@ __global__ __launch_bounds__(256) void k_bin_scatter(long long np, const double 
 }
 
 // slot order -> original order for selected planes (download)
-__global__ __launch_bounds__(256) void k_unsort(long long np, int nf, const double *src, long long src_stride,
-                                                double *dst, const int *pid) {
+template <class E>
+__global__ __launch_bounds__(256) void k_unsort(long long np, int nf, const E *src, long long src_stride, E *dst,
+                                                const int *pid) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     const int o = pid[p];

@@ -1,0 +1,174 @@
+// hip_explicit_usl.cpp -- HipExplicitUSL: the explicit USL step (explicit_usl.cpp:48-87) on
+// the GPU through the C-ABI of include/mpm_gpu.h.
+//
+// init reads the ExplicitUSL properties (explicit_usl.cpp:25-44). Job::init calls the
+// solver before the grid and bodies, so the device state is created on the first step:
+// the plugin set is checked (anything the GPU path does not implement is refused with the
+// reference's convention, a message on stderr and exit(0)), the AoS point arrays are
+// converted to SoA planes and uploaded once. A step is queued; the queued steps run as one
+// mpm_gpu_step call when the host arrays are needed (syncHost: a frame, the end of the
+// run, a saveState), so the point state is copied back only then.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <iostream>
+#include <stdexcept>
+
+#include "mpm_gpu.h"
+#include "mpm_host.hpp"
+
+namespace mpmhost {
+
+static void refuse(const char *func, const std::string &what) {
+    fprintf(stderr, "%s:%s: %s\n", __FILE__, func, what.c_str());
+    exit(0);
+}
+
+static void check(int rc, const char *what) {
+    if (rc != MPM_OK) throw std::runtime_error(std::string(what) + ": " + mpm_gpu_last_error());
+}
+
+HipExplicitUSL::~HipExplicitUSL() {
+    if (handle) mpm_gpu_destroy(handle);
+}
+
+void HipExplicitUSL::init(Job *) {
+    cpdi_spec = 1;     // DefaultBody::CPDI_ON
+    contact_spec = 1;  // Contact::IMPLICIT
+    if (int_props.size() >= 1) cpdi_spec = int_props[0];
+    if (int_props.size() >= 2) contact_spec = int_props[1];
+    if (int_props.size() >= 3) precision = int_props[2];
+    if (int_props.size() >= 4) sort_interval = int_props[3];
+    printf("Solver properties (cpdi_spec = %i, contact_spec = %i, precision = %i).\n", cpdi_spec, contact_spec,
+           precision);
+    std::cout << "Solver Initialized." << std::endl;
+}
+
+void HipExplicitUSL::upload(Job *job) {
+    if (job->JOB_TYPE != Job::JOB_2D && job->JOB_TYPE != Job::JOB_3D)
+        refuse(__func__, "HipExplicitUSL supports TYPE = 2 and 3 only.");
+    auto *grid = dynamic_cast<CartesianLinear *>(job->grid.get());
+    if (!grid) refuse(__func__, "HipExplicitUSL needs the CartesianLinear grid.");
+    if (precision != MPM_PREC_FP64 && precision != MPM_PREC_FP32) refuse(__func__, "precision must be 0 or 1.");
+    const int dim = job->DIM;
+    mpm_grid_desc g{};
+    g.dim = dim;
+    for (int d = 0; d < dim; d++) {
+        g.L[d] = grid->L[d];
+        g.N[d] = grid->N[d];
+    }
+    g.cpdi_spec = cpdi_spec;
+    g.contact_spec = contact_spec;
+    for (int d = 0; d < 3; d++) g.gravity[d] = job->driver->gravity[d];
+
+    std::vector<mpm_body_desc> bd;
+    for (auto &bp : job->bodies) {
+        if (!bp || !bp->points) refuse(__func__, "every body needs points.");
+        mpm_body_desc b{};
+        b.n_points = (int64_t)bp->points->size();
+        if (bp->activeMaterial) {
+            auto *mat = dynamic_cast<IsotropicLinearElasticity *>(bp->material.get());
+            if (!mat) refuse(__func__, "HipExplicitUSL supports the IsotropicLinearElasticity material only.");
+            b.has_material = 1;
+            b.E = mat->E;
+            b.nu = mat->nu;
+        }
+        if (bp->activeBoundary) {
+            if (!dynamic_cast<CartesianBox *>(bp->boundary.get()))
+                refuse(__func__, "HipExplicitUSL supports the CartesianBox boundary only.");
+            b.has_boundary = 1;
+        }
+        bd.push_back(b);
+    }
+    if (bd.size() > MPM_MAX_BODIES_PER_SIM) refuse(__func__, "at most 4 bodies per simulation.");
+    std::vector<mpm_contact_desc> cd;
+    for (auto &cp : job->contacts) {
+        auto *c = dynamic_cast<ContactHuang *>(cp.get());
+        if (!c) refuse(__func__, "HipExplicitUSL supports ContactHuang contacts only.");
+        cd.push_back(mpm_contact_desc{c->bodyIDs[0], c->bodyIDs[1], c->mu_f});
+    }
+    check(mpm_gpu_create(&g, bd.data(), (int)bd.size(), cd.empty() ? nullptr : cd.data(), (int)cd.size(), precision,
+                         &handle),
+          "mpm_gpu_create");
+    check(mpm_gpu_set_sort_interval(handle, sort_interval), "mpm_gpu_set_sort_interval");
+    for (size_t b = 0; b < job->bodies.size(); b++) {
+        const Points &P = *job->bodies[b]->points;
+        const size_t n = P.size();
+        std::vector<double> x(dim * n), v(dim * n), T(6 * n);
+        std::vector<int32_t> act(n);
+        static const int t6[6] = {0, 4, 8, 1, 2, 5};  // xx yy zz xy xz yz in a row-major 3x3
+        for (size_t i = 0; i < n; i++) {
+            for (int d = 0; d < dim; d++) {
+                x[d * n + i] = P.x[3 * i + d];
+                v[d * n + i] = P.x_t[3 * i + d];
+            }
+            for (int k = 0; k < 6; k++) T[k * n + i] = P.T[9 * i + t6[k]];
+            act[i] = P.active[i];
+        }
+        check(mpm_gpu_upload(handle, (int)b, x.data(), v.data(), T.data(), P.v.data(), P.m.data(), P.extent.data(),
+                             act.data()),
+              "mpm_gpu_upload");
+    }
+}
+
+void HipExplicitUSL::flush(Job *job) {
+    if (pending > 0) {
+        long long left = pending;
+        while (left > 0) {
+            const int k = (int)std::min<long long>(left, 1 << 30);
+            check(mpm_gpu_step(handle, job->dt, k), "mpm_gpu_step");
+            left -= k;
+        }
+        pending = 0;
+        host_stale = true;
+    }
+}
+
+void HipExplicitUSL::step(Job *job) {
+    if (!handle) upload(job);
+    pending++;
+}
+
+void HipExplicitUSL::syncHost(Job *job) {
+    if (!handle) return;
+    flush(job);
+    if (!host_stale) return;
+    const int dim = job->DIM;
+    for (size_t b = 0; b < job->bodies.size(); b++) {
+        Points &P = *job->bodies[b]->points;
+        const size_t n = P.size();
+        if (n == 0) continue;
+        std::vector<double> x(dim * n), v(dim * n), T(6 * n), V(n), u(dim * n), L(9 * n);
+        std::vector<int32_t> act(n);
+        check(mpm_gpu_download(handle, (int)b, x.data(), v.data(), T.data(), V.data(), act.data()), "mpm_gpu_download");
+        check(mpm_gpu_download_aux(handle, (int)b, u.data(), L.data()), "mpm_gpu_download_aux");
+        static const int r6[6] = {0, 1, 2, 0, 0, 1}, c6[6] = {0, 1, 2, 1, 2, 2};
+        for (size_t i = 0; i < n; i++) {
+            for (int d = 0; d < dim; d++) {
+                P.x[3 * i + d] = x[d * n + i];
+                P.x_t[3 * i + d] = v[d * n + i];
+                P.u[3 * i + d] = u[d * n + i];
+            }
+            for (int d = 0; d < 3; d++) P.mx_t[3 * i + d] = P.m[i] * P.x_t[3 * i + d];
+            for (int k = 0; k < 6; k++) {
+                P.T[9 * i + 3 * r6[k] + c6[k]] = T[k * n + i];
+                P.T[9 * i + 3 * c6[k] + r6[k]] = T[k * n + i];
+            }
+            for (int k = 0; k < 9; k++) P.L[9 * i + k] = L[k * n + i];
+            P.v[i] = V[i];
+            P.active[i] = act[i];
+        }
+    }
+    host_stale = false;
+}
+
+// The reference's ExplicitUSL returns "err" / 0 here (explicit_usl.cpp:91-99); this solver
+// brings the host arrays up to date so a serializer can save them.
+std::string HipExplicitUSL::saveState(Job *job, Serializer *, std::string) {
+    syncHost(job);
+    return "err";
+}
+
+int HipExplicitUSL::loadState(Job *, Serializer *, std::string) { return 0; }
+
+}  // namespace mpmhost
