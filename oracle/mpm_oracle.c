@@ -15,14 +15,15 @@
  * Build with -ffp-contract=off so no multiply-add is fused (the reference is built
  * without FMA contraction).
  *
- * Pins (tests/test_oracle_*.py): the reference ships no golden vectors for this path
+ * Pins (tests/test_oracle_pins.py): the reference ships no golden vectors for this path
  * (its examples hold inputs only), so every function here is pinned by closed forms and
  * invariants chosen so that a dropped term, wrong sign, wrong index or transposed operand
  * fails: brute-force element search, partition of unity and linear reproduction of S and
- * gradS, the CPDI-OFF gradient against finite differences, mass/momentum conservation of
- * P2G, free fall and rigid translation trajectories in closed form, hand-computed Jaumann
- * updates (uniaxial, pure spin), two-node contact closed forms, and the algebra identities
- * of src/tests/algebra_test.cpp.
+ * gradS, hand values of the shipped blockdrop2D fixture, the CPDI-OFF gradient against
+ * finite differences, mass/momentum conservation of P2G, free fall and rigid translation
+ * trajectories in closed form, a linear velocity field (L, density, Hooke stress),
+ * hand-computed Jaumann updates (uniaxial, volumetric, shear, spin), the CartesianBox mask,
+ * and ContactHuang's momentum conservation, normals and friction limits.
  */
 #define _GNU_SOURCE
 #include "mpm_oracle.h"

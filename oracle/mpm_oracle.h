@@ -15,6 +15,18 @@
  *
  * Layout: vectors are AoS with 3 doubles per point/node (the third is 0 in 2D),
  * tensors are row-major 3x3 (9 doubles), as in the reference's arrays.
+ *
+ * Pins (tests/test_oracle_pins.py): the reference ships no golden vectors for this path
+ * and cannot be compiled here (it needs Eigen 3), so each function is pinned by an
+ * independent statement -- ora_which_element / ora_in_domain by brute force and edge
+ * cases; ora_basis_value / ora_basis_gradient by hand products, partition of unity,
+ * linear reproduction, central differences and the 0/0 NaN; ora_get_map by hand values
+ * of the shipped blockdrop2D fixture and the separable identity; ora_isolin_update by
+ * uniaxial, volumetric, shear and spin closed forms; ora_step by free fall, rigid
+ * translation, a linear velocity field, conservation laws, the CartesianBox mask and
+ * ContactHuang's momentum, normal and friction limits; the threaded twin against the
+ * serial path. Parity unpinned: bitwise multi-step agreement with the reference binary's
+ * own summation order (that needs the reference itself).
  */
 #ifndef MPM_ORACLE_H
 #define MPM_ORACLE_H
