@@ -16,6 +16,12 @@ The JSON line also carries
                 time (HIP events on the library's stream), against 8 TB/s
   cpu_baseline  the oracle's threaded twin of ThreadPoolExplicitUSL (oracle/, -O3
                 -march=native) on a bounded sample of the same workload, on this host
+  batched       512 independent 2D block drops (16^2..256^2 grids) in one handle, precision
+                form, per-simulation dt, HIP-graph sort periods, autotuned launch shapes:
+                solves/s and particle-updates/s against the same simulations one at a time
+  served        a Poisson stream of 3D block drops / collisions admitted into a slot pool
+                sized by device footprint, continuous batching: p50/p99 latency, solves/s,
+                particle-updates/s against the same requests one at a time
 """
 from __future__ import annotations
 
@@ -24,6 +30,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -118,6 +126,94 @@ def run_form(w, form, args):
     }
 
 
+def run_batched(args):
+    """Batched 2D ensemble: 512 block drops (seeded) in one handle, precision form,
+    per-simulation dt, graph-replayed sort periods, autotuned launch shapes, `ens_steps`
+    steps each; against the same simulations run one at a time (timed on a sample)."""
+    from mpm_3d_amd import workloads as W
+    from mpm_3d_amd.gpu import MPM_OPT_AUTOTUNE, MPM_OPT_GRAPH, MPM_OPT_GRAPH_CAPTURES, MPM_OPT_GRID_WGS, \
+        MPM_OPT_PASS_GRID, MPM_PREC_FP32, GpuSim
+    ws = W.ensemble_2d(n_sims=512, steps=args.ens_steps)
+    npts = sum(w.n_points for w in ws)
+    sim = GpuSim(ws, precision=MPM_PREC_FP32)
+    sim.set_option(MPM_OPT_GRAPH, 1)
+    sim.set_option(MPM_OPT_AUTOTUNE, 1)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step_batch([w.dt for w in ws], args.ens_steps)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    tuned = {"pass_grid": sim.get_option(MPM_OPT_PASS_GRID), "grid_wgs": sim.get_option(MPM_OPT_GRID_WGS),
+             "graph_captures": sim.get_option(MPM_OPT_GRAPH_CAPTURES)}
+    sim.close()
+    print(f"[bench] batched: {len(ws)} sims, {npts} points, {args.ens_steps} steps in {wall:.2f} s", file=sys.stderr,
+          flush=True)
+    # one at a time: every 32nd simulation on its own handle (same form, same steps)
+    sample = ws[::32]
+    t1 = time.perf_counter()
+    for w in sample:
+        one = GpuSim(w, precision=MPM_PREC_FP32)
+        one.step(w.dt, args.ens_steps)
+        one.synchronize()
+        one.close()
+    alone = time.perf_counter() - t1
+    alone_pts = sum(w.n_points for w in sample)
+    out = {
+        "workload": f"ensemble_2d: 512 block drops, grid 16^2..256^2, {npts} points total, {args.ens_steps} steps each",
+        "solves_per_s": len(ws) / wall,
+        "particle_updates_per_s": npts * args.ens_steps / wall,
+        "wall_s": wall,
+        "tuned": tuned,
+        "one_at_a_time": {"sample": len(sample), "solves_per_s": len(sample) / alone,
+                          "particle_updates_per_s": alone_pts * args.ens_steps / alone,
+                          "note": "each sampled simulation created, uploaded and stepped alone"},
+    }
+    out["speedup_solves"] = out["solves_per_s"] / out["one_at_a_time"]["solves_per_s"]
+    out["speedup_particle_updates"] = out["particle_updates_per_s"] / out["one_at_a_time"]["particle_updates_per_s"]
+    return out
+
+
+def run_served(args):
+    """Served 3D stream: `serve_n` requests (seeded block drops / two-block collisions,
+    N^3 grids 32..128, 1e4..2e6 points, 500..5000 steps), Poisson arrivals at `serve_rate`
+    per second, continuous batching in a slot pool sized by device footprint; latency =
+    completion - arrival. One-at-a-time: a sample of the same requests run alone."""
+    from mpm_3d_amd import serve as S
+    from mpm_3d_amd import workloads as W
+    from mpm_3d_amd.gpu import MPM_PREC_FP32
+    reqs = W.serve_requests(args.serve_n)
+    caps = (2_000_000, 1_000_000)
+    slots, per = S.slots_for_budget(3, 128, caps, args.serve_budget_gb * 1e9)
+    slots = min(slots, args.serve_n)
+    pool = S.SlotPool(slots, dim=3, N_cap=128, body_caps=caps, precision=MPM_PREC_FP32)
+    arrivals = S.poisson_arrivals(len(reqs), args.serve_rate)
+    res = S.run_stream(pool, reqs, arrivals, chunk_max=64)
+    pool.close()
+    lat = np.array([f - a for f, a in zip(res["finish"], res["arrival"])])
+    pu = sum(w.n_points * st for w, st in reqs)
+    print(f"[bench] served: {len(reqs)} requests on {slots} slots in {res['makespan']:.2f} s, "
+          f"p50 {np.percentile(lat, 50):.3f} s p99 {np.percentile(lat, 99):.3f} s", file=sys.stderr, flush=True)
+    sample = reqs[::8]
+    t1 = time.perf_counter()
+    for w, st in sample:
+        S.run_alone(w, st)
+    alone = time.perf_counter() - t1
+    out = {
+        "workload": f"{len(reqs)} requests (block drop / two-block collision, N^3 = 32^3..128^3, "
+                    f"{min(w.n_points for w, _ in reqs)}..{max(w.n_points for w, _ in reqs)} points, "
+                    f"500..5000 steps), Poisson arrivals at {args.serve_rate}/s",
+        "slots": slots, "slot_footprint_bytes": per,
+        "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+        "solves_per_s": len(reqs) / res["makespan"],
+        "particle_updates_per_s": pu / res["makespan"],
+        "makespan_s": res["makespan"],
+        "one_at_a_time": {"sample": len(sample), "solves_per_s": len(sample) / alone,
+                          "particle_updates_per_s": sum(w.n_points * st for w, st in sample) / alone},
+    }
+    out["speedup_particle_updates"] = out["particle_updates_per_s"] / out["one_at_a_time"]["particle_updates_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,7 +223,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--sort-interval", type=int, default=16)
-    ap.add_argument("--fused", action="store_true", help="one fused G2P+P2G pass per step (FP64 form only)")
+    ap.add_argument("--fused", action="store_true", help="one fused G2P+P2G pass per step instead of two passes")
+    ap.add_argument("--legs", default="column,batched,served",
+                    help="comma list of: column (16M column, both forms), batched (512-sim 2D ensemble), served")
+    ap.add_argument("--ens-steps", type=int, default=2000)
+    ap.add_argument("--serve-n", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 10 launches) used to calibrate counters")
     args = ap.parse_args()
@@ -173,6 +275,11 @@ def main():
     }
     if len(res) == 2:
         out["fp32_vs_fp64"] = round(res["fp32"]["value"] / res["fp64"]["value"], 3)
+    legs = set(args.legs.split(","))
+    if "batched" in legs:
+        out["batched"] = run_batched(args)
+    if "served" in legs:
+        out["served"] = run_served(args)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         print(f"[bench] cpu baseline with {threads} threads", file=sys.stderr, flush=True)

@@ -107,6 +107,23 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
  * (explicit_usl.cpp:48-87) with DefaultDriver gravity (default_driver.cpp:46-86). */
 int mpm_gpu_step(mpm_gpu_handle *h, double dt, int nsteps);
 
+/* Serving a stream of simulations from one batch handle: each simulation of the handle
+ * is a slot whose point, body, contact and grid sizes at creation are its capacity.
+ * mpm_gpu_load_sim reconfigures slot `sim` for a new simulation `d` (grid L and N <= the
+ * slot's N, up to its bodies and contacts, n_points per body <= the body's capacity),
+ * deactivating whatever the slot held; then mpm_gpu_upload fills each body (n_points
+ * entries). mpm_gpu_clear_sim empties a slot. Other slots keep their state and can keep
+ * stepping between these calls. Replaces building a new Job per request
+ * (config.cpp:176-206 + Job::init, job.hpp:76-106); no counterpart in the reference. */
+int mpm_gpu_load_sim(mpm_gpu_handle *h, int sim, const mpm_sim_desc *d);
+int mpm_gpu_clear_sim(mpm_gpu_handle *h, int sim);
+
+/* Batched form of mpm_gpu_step: every simulation of the handle advances nsteps steps,
+ * simulation s with its own time step dt[s] (nsim entries, in create order). Replaces
+ * ExplicitUSL::step of each simulation's Job (explicit_usl.cpp:48-87), in one launch
+ * sequence for all. */
+int mpm_gpu_step_batch(mpm_gpu_handle *h, const double *dt, int nsim, int nsteps);
+
 /* Download one body in its original point order. Any pointer may be NULL.
  * Replaces reads of Points::x, x_t, T, v, active (mpm_objects.hpp:137-159). */
 int mpm_gpu_download(mpm_gpu_handle *h, int body, double *x, double *v, double *T,
@@ -134,7 +151,21 @@ int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
  *                          P2G; 0 (default): separate G2P and P2G passes */
 #define MPM_OPT_SORT_INTERVAL 1
 #define MPM_OPT_FUSED_PASS 2
+/*   MPM_OPT_GRAPH          1: replay each sort period (a sort + sort_interval steps) as
+ *                          a captured HIP graph (split passes, timing off); 0 (default)
+ *   MPM_OPT_AUTOTUNE       1: the next step call first times candidate launch shapes
+ *                          (persistent particle-pass grid, grid-update grid) on its own
+ *                          first steps and keeps the fastest; those steps count
+ *   MPM_OPT_PASS_GRID      workgroups of the persistent particle passes (default 2048)
+ *   MPM_OPT_GRID_WGS       workgroups of the grid update (default 2048)
+ *   MPM_OPT_GRAPH_CAPTURES (get only) graphs captured so far */
+#define MPM_OPT_GRAPH 3
+#define MPM_OPT_AUTOTUNE 4
+#define MPM_OPT_PASS_GRID 5
+#define MPM_OPT_GRID_WGS 6
+#define MPM_OPT_GRAPH_CAPTURES 7
 int mpm_gpu_set_option(mpm_gpu_handle *h, int key, int value);
+int mpm_gpu_get_option(mpm_gpu_handle *h, int key, int *value);
 
 /* Kernel timing with HIP events on the handle's stream: enable, then read the number of
  * timed launches and their summed milliseconds per kernel class

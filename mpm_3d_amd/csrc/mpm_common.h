@@ -170,21 +170,24 @@ struct KArgs {
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // sim-local node (i,j,k), -1 <= i <= N+1, -> offset inside the sim's node-block storage.
-// Storage is shifted by one node so the zero-weight slots of a 3-node window that hang
-// over the domain edge (nodes -1 and N+1) have a home; those padding nodes stay zero.
+// Storage is shifted by two nodes: the zero-weight slots of a 3-node window that hang
+// over the domain edge (nodes -1 and N+1) have a home (those padding nodes stay zero),
+// and the LDS patch of particle block pb, nodes pb*PB-2 .. pb*PB+P-3, starts on a
+// node-block boundary, so staging and flushing it touch whole 64-node blocks.
+constexpr int NSHIFT = 2;
 template <int DIM>
 __host__ __device__ __forceinline__ long long node_addr(const SimDev &s, const int *ijk) {
     constexpr int NB = Cfg<DIM>::NB, L = Cfg<DIM>::LNB;
-    const int i0 = ijk[0] + 1;
+    const int i0 = ijk[0] + NSHIFT;
     long long blk = i0 >> L;
     int loc = i0 & (NB - 1);
     if (DIM >= 2) {
-        const int i1 = ijk[1] + 1;
+        const int i1 = ijk[1] + NSHIFT;
         blk += (long long)(i1 >> L) * s.NBn[0];
         loc += (i1 & (NB - 1)) << L;
     }
     if (DIM == 3) {
-        const int i2 = ijk[DIM - 1] + 1;
+        const int i2 = ijk[DIM - 1] + NSHIFT;
         blk += (long long)(i2 >> L) * s.NBn[0] * s.NBn[1];
         loc += (i2 & (NB - 1)) << (2 * L);
     }

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -87,7 +88,7 @@ int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
                 s.ftol[d] = std::ldexp((double)g.N[d], -48);
                 s.hf[d] = (float)s.h[d];
                 s.invhf[d] = (float)s.invh[d];
-                s.NBn[d] = (int)ceil_div(g.N[d] + 3, NB);  // nodes -1..N+1
+                s.NBn[d] = (int)ceil_div(g.N[d] + 2 + NSHIFT, NB);  // storage of nodes -NSHIFT..N+1
                 s.PNB[d] = (int)ceil_div(g.N[d] + 1, PB);
                 nblk *= s.NBn[d];
                 pnb *= s.PNB[d];
@@ -209,12 +210,23 @@ struct mpm_gpu_handle {
     double *stage = nullptr;  // download staging
     long long stage_elems = 0;
     int par = 0;
+    int buf = 0;  // which of the two particle buffers is current (toggled by every sort)
     int sort_interval = 16;
+    int pass_grid = 2048;  // persistent workgroups of the particle passes
+    int grid_wgs = 2048;   // workgroups of the grid update
+    bool autotune = false, tuned = false;
+    bool use_graph = false;
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // per (parity, buffer)
+    int graph_len[4] = {0, 0, 0, 0};
+    int graph_captures = 0;
     bool sorted = false;  // slots are in particle-block order (cleared by upload)
     int since_sort = 0;   // steps advanced since the last sort
     int fused = 0;  // 1: one particle pass per step (G2P then P2G); 0: two passes (default: faster on gfx950)
     bool sims_dirty = true;
     std::vector<char> uploaded;
+    std::vector<long long> count;  // points in use per body (<= its capacity BodyDev::n)
+    std::vector<std::array<int, 3>> grid_cap;  // per sim: N at creation (node storage size)
+    std::vector<int> contact_cap;              // per sim: contacts at creation
     std::vector<std::vector<double>> x0;  // per body, dim planes (for displacement)
     bool timing = false;
     std::vector<Timer> timers;
@@ -296,6 +308,8 @@ int dalloc(T **p, long long n) {
     return MPM_OK;
 }
 
+void swap_buffers(mpm_gpu_handle *h);
+
 template <int DIM, class PT>
 int launch_sort(mpm_gpu_handle *h) {
     const long long np = h->lo.np, nk = h->lo.nkeys;
@@ -336,60 +350,82 @@ int launch_sort(mpm_gpu_handle *h) {
     if constexpr (PT::FP32) {
         hipLaunchKernelGGL((k_bin_scatter<uint32_t, PW<DIM>::N>), dim3(gp), dim3(256), 0, h->stream, np, h->Pw, h->Pw2,
                            h->act, h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff);
-        std::swap(h->Pw, h->Pw2);
     } else {
         hipLaunchKernelGGL((k_bin_scatter<double, PF<DIM>::N>), dim3(gp), dim3(256), 0, h->stream, np, h->P, h->P2,
                            h->act, h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff);
-        std::swap(h->P, h->P2);
     }
-    std::swap(h->act, h->act2);
-    std::swap(h->pid, h->pid2);
+    swap_buffers(h);
     HIP_TRY(hipGetLastError());
     return timer_end(h, t);
 }
 
-constexpr int PASS_GRID = 2048;  // persistent workgroups for the particle passes
-
 template <int DIM, class PT, bool G2P, bool P2G>
 void launch_particles(mpm_gpu_handle *h, int want_L) {
     KArgs A = kargs(h);
+    const dim3 grid(h->pass_grid);
     if (h->lo.any_b1)
-        hipLaunchKernelGGL((k_particles<DIM, G2P, P2G, true, PT>), dim3(PASS_GRID), dim3(256), 0, h->stream, A, want_L);
+        hipLaunchKernelGGL((k_particles<DIM, G2P, P2G, true, PT>), grid, dim3(256), 0, h->stream, A, want_L);
     else
-        hipLaunchKernelGGL((k_particles<DIM, G2P, P2G, false, PT>), dim3(PASS_GRID), dim3(256), 0, h->stream, A,
-                           want_L);
+        hipLaunchKernelGGL((k_particles<DIM, G2P, P2G, false, PT>), grid, dim3(256), 0, h->stream, A, want_L);
 }
 
-// Step sequence with the list parity `par`: a P2G half marks node blocks into list[par];
-// k_grid(par) consumes it and clears list[par^1] for the next P2G.
-//   sort; P2G(par); { grid(par); par^=1; [sort]; G2P+P2G(par) } ...; grid(par); par^=1; G2P
 template <int DIM, class PT>
-int launch_step(mpm_gpu_handle *h, int nsteps) {
-    Timer t{};
-    int rc = MPM_OK;
-    // sort when the order is stale: after an upload, when the interval has passed, or at
-    // every call with interval 0; a driver stepping one step per call pays no extra sort
-    if (!h->sorted || h->sort_interval == 0 || h->since_sort >= h->sort_interval) {
-        if ((rc = launch_sort<DIM, PT>(h))) return rc;
-        h->sorted = true;
-        h->since_sort = 0;
-    }
-    rc = timer_begin(h, KC_P2G, t);
+void launch_grid(mpm_gpu_handle *h, bool want_stats) {
+    KArgs Ag = kargs(h);
+    if (want_stats) hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(64), 0, h->stream, h->stats, 4);
+    else Ag.stats = nullptr;
+    hipLaunchKernelGGL((k_grid<DIM, typename PT::GO>), dim3(h->grid_wgs), dim3(256), 0, h->stream, Ag);
+}
+
+// sort when the order is stale: after an upload, when the interval has passed, or at the
+// start of every call with interval 0
+template <int DIM, class PT>
+int sort_if_due(mpm_gpu_handle *h, bool first_of_call) {
+    const bool due = !h->sorted || (h->sort_interval == 0 && first_of_call) ||
+                     (h->sort_interval > 0 && h->since_sort >= h->sort_interval);
+    if (!due) return MPM_OK;
+    const int rc = launch_sort<DIM, PT>(h);
     if (rc) return rc;
+    h->sorted = true;
+    h->since_sort = 0;
+    return MPM_OK;
+}
+
+// One step with separate passes: [sort] P2G(par) grid(par) G2P. P2G marks node blocks
+// into list[par]; k_grid consumes it and clears list[par^1] for the next step.
+template <int DIM, class PT>
+int split_step(mpm_gpu_handle *h, bool first, bool last) {
+    Timer t{};
+    int rc = sort_if_due<DIM, PT>(h, first);
+    if (rc) return rc;
+    if ((rc = timer_begin(h, KC_P2G, t))) return rc;
+    launch_particles<DIM, PT, false, true>(h, 0);
+    if ((rc = timer_end(h, t))) return rc;
+    if ((rc = timer_begin(h, KC_GRID, t))) return rc;
+    launch_grid<DIM, PT>(h, last);
+    if ((rc = timer_end(h, t))) return rc;
+    h->par ^= 1;
+    if ((rc = timer_begin(h, KC_G2P, t))) return rc;
+    launch_particles<DIM, PT, true, false>(h, last ? 1 : 0);
+    if ((rc = timer_end(h, t))) return rc;
+    h->since_sort++;
+    HIP_TRY(hipGetLastError());
+    return MPM_OK;
+}
+
+// The fused sequence: sort; P2G; { grid; par^=1; [sort]; G2P+P2G } ...; grid; G2P
+template <int DIM, class PT>
+int fused_steps(mpm_gpu_handle *h, int nsteps) {
+    Timer t{};
+    int rc = sort_if_due<DIM, PT>(h, true);
+    if (rc) return rc;
+    if ((rc = timer_begin(h, KC_P2G, t))) return rc;
     launch_particles<DIM, PT, false, true>(h, 0);
     if ((rc = timer_end(h, t))) return rc;
     for (int k = 0; k < nsteps; k++) {
         const bool last = k == nsteps - 1;
         if ((rc = timer_begin(h, KC_GRID, t))) return rc;
-        {
-            KArgs Ag = kargs(h);
-            if (last) {
-                hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(64), 0, h->stream, h->stats, 4);
-            } else {
-                Ag.stats = nullptr;
-            }
-            hipLaunchKernelGGL((k_grid<DIM, typename PT::GO>), dim3(PASS_GRID), dim3(256), 0, h->stream, Ag);
-        }
+        launch_grid<DIM, PT>(h, last);
         if ((rc = timer_end(h, t))) return rc;
         h->par ^= 1;
         h->since_sort++;
@@ -402,20 +438,135 @@ int launch_step(mpm_gpu_handle *h, int nsteps) {
                 if ((rc = launch_sort<DIM, PT>(h))) return rc;
                 h->since_sort = 0;
             }
-            if (h->fused && !PT::FP32) {
-                if ((rc = timer_begin(h, KC_PASS, t))) return rc;
-                launch_particles<DIM, Prec64, true, true>(h, 0);
-                if ((rc = timer_end(h, t))) return rc;
-            } else {
-                if ((rc = timer_begin(h, KC_G2P, t))) return rc;
-                launch_particles<DIM, PT, true, false>(h, 0);
-                if ((rc = timer_end(h, t))) return rc;
-                if ((rc = timer_begin(h, KC_P2G, t))) return rc;
-                launch_particles<DIM, PT, false, true>(h, 0);
-                if ((rc = timer_end(h, t))) return rc;
-            }
+            if ((rc = timer_begin(h, KC_PASS, t))) return rc;
+            launch_particles<DIM, PT, true, true>(h, 0);
+            if ((rc = timer_end(h, t))) return rc;
         }
         HIP_TRY(hipGetLastError());
+    }
+    return MPM_OK;
+}
+
+void swap_buffers(mpm_gpu_handle *h) {
+    std::swap(h->P, h->P2);
+    std::swap(h->Pw, h->Pw2);
+    std::swap(h->act, h->act2);
+    std::swap(h->pid, h->pid2);
+    h->buf ^= 1;
+}
+
+// One sort period (sort + sort_interval split steps) as a HIP graph. The captured kernels
+// bake the list parity and the particle buffer they start from, so a graph is kept per
+// (parity, buffer) and replayed only from that state; replaying advances the host-side
+// state exactly as capturing did.
+template <int DIM, class PT>
+int graph_period(mpm_gpu_handle *h) {
+    const int S = h->sort_interval;
+    const int key = h->par * 2 + h->buf;
+    if (h->graph[key] && h->graph_len[key] == S) {
+        HIP_TRY(hipGraphLaunch(h->graph[key], h->stream));
+        swap_buffers(h);  // the period's sort
+        if (S & 1) h->par ^= 1;
+        h->since_sort = S;
+        return MPM_OK;
+    }
+    if (h->graph[key]) {
+        HIP_TRY(hipGraphExecDestroy(h->graph[key]));
+        h->graph[key] = nullptr;
+    }
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = MPM_OK;
+    for (int j = 0; j < S && rc == MPM_OK; j++) rc = split_step<DIM, PT>(h, false, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(MPM_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) return fail(MPM_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e2));
+    h->graph[key] = ex;
+    h->graph_len[key] = S;
+    h->graph_captures++;
+    HIP_TRY(hipGraphLaunch(ex, h->stream));  // host state was advanced while capturing
+    return MPM_OK;
+}
+
+// Launch-shape autotuner: times whole steps (HIP events) for candidate persistent-grid
+// sizes of the particle passes, then of the grid kernel, and keeps the fastest. The
+// steps it times are real steps of the run (results do not depend on launch shape beyond
+// atomic summation order). Returns the number of steps consumed.
+template <int DIM, class PT>
+int autotune(mpm_gpu_handle *h, int budget, int *used) {
+    *used = 0;
+    static const int pass_cand[] = {512, 1024, 2048, 4096, 8192};
+    static const int grid_cand[] = {512, 2048, 8192};
+    const int reps = 2;
+    const int need = reps * (5 + 3) + 1;
+    if (budget < need) return MPM_OK;  // too short a call to tune; tune on a later one
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const bool timing = h->timing;
+    h->timing = false;
+    int rc = MPM_OK;
+    auto measure = [&](float *ms) -> int {
+        if (hipEventRecord(e0, h->stream) != hipSuccess) return MPM_ERR_HIP;
+        for (int r = 0; r < reps; r++) {
+            const int rr = split_step<DIM, PT>(h, *used == 0, false);
+            if (rr) return rr;
+            (*used)++;
+        }
+        if (hipEventRecord(e1, h->stream) != hipSuccess) return MPM_ERR_HIP;
+        if (hipEventSynchronize(e1) != hipSuccess) return MPM_ERR_HIP;
+        return hipEventElapsedTime(ms, e0, e1) == hipSuccess ? MPM_OK : MPM_ERR_HIP;
+    };
+    float best = 1e30f, ms = 0;
+    int best_pass = h->pass_grid, best_grid = h->grid_wgs;
+    for (int c : pass_cand) {
+        h->pass_grid = c;
+        if ((rc = measure(&ms))) break;
+        if (ms < best) { best = ms; best_pass = c; }
+    }
+    h->pass_grid = best_pass;
+    best = 1e30f;
+    for (int c : grid_cand) {
+        if (rc) break;
+        h->grid_wgs = c;
+        if ((rc = measure(&ms))) break;
+        if (ms < best) { best = ms; best_grid = c; }
+    }
+    h->grid_wgs = best_grid;
+    h->tuned = true;
+    h->timing = timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return fail(rc, "autotune: a timed step failed");
+    return MPM_OK;
+}
+
+template <int DIM, class PT>
+int launch_step(mpm_gpu_handle *h, int nsteps) {
+    if (h->fused) return fused_steps<DIM, PT>(h, nsteps);
+    int k = 0, rc = MPM_OK;
+    if (h->autotune && !h->tuned) {
+        int used = 0;
+        if ((rc = autotune<DIM, PT>(h, nsteps - 1, &used))) return rc;
+        k = used;
+    }
+    const bool graphs = h->use_graph && !h->timing && h->sort_interval > 0;
+    while (k < nsteps) {
+        const bool last = k == nsteps - 1;
+        if (graphs && h->sorted && h->since_sort >= h->sort_interval && nsteps - 1 - k >= h->sort_interval) {
+            if ((rc = graph_period<DIM, PT>(h))) return rc;
+            k += h->sort_interval;
+            continue;
+        }
+        if ((rc = split_step<DIM, PT>(h, k == 0, last))) return rc;
+        k++;
     }
     return MPM_OK;
 }
@@ -519,6 +670,12 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     }
     if (e != hipSuccess) return cleanup(fail(MPM_ERR_HIP, "initialising device state: %s", hipGetErrorString(e)));
     h->uploaded.assign(L.bodies.size(), 0);
+    for (const SimDev &sd : L.sims) {
+        h->grid_cap.push_back({sd.N[0], sd.N[1], sd.N[2]});
+        h->contact_cap.push_back(sd.ncontact);
+    }
+    h->count.resize(L.bodies.size());
+    for (size_t b = 0; b < L.bodies.size(); b++) h->count[b] = L.bodies[b].n;
     h->x0.resize(L.bodies.size());
     h->sims_dirty = true;
 #undef TRY
@@ -563,41 +720,33 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
                    const double *V, const double *m, const double *extent, const int32_t *active) {
     if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
     if (body < 0 || body >= (int)h->lo.bodies.size()) return fail(MPM_ERR_ARG, "body %d out of range", body);
-    if (!x || !v || !V || !m || !extent || !active) return fail(MPM_ERR_ARG, "x, v, V, m, extent, active are required");
     const BodyDev &B = h->lo.bodies[body];
     const int dim = h->lo.dim;
-    const long long n = B.n, np = h->lo.np;
-    if (n == 0) {
-        h->uploaded[body] = 1;
-        return MPM_OK;
-    }
+    const long long n = h->count[body], np = h->lo.np;
+    if (n > 0 && (!x || !v || !V || !m || !extent || !active))
+        return fail(MPM_ERR_ARG, "x, v, V, m, extent, active are required");
     // The sorted slots of other bodies are interleaved with this body's, so an upload
-    // writes the staging buffer in original order and re-sorts from there: bring every
-    // body back to original order first.
+    // brings every plane back to original order on the device (second buffer), writes
+    // this body's range there and re-sorts before the next step.
     const bool f32 = h->precision == MPM_PREC_FP32;
     const int nf = f32 ? (dim == 3 ? PW<3>::N : PW<2>::N) : (dim == 3 ? PF<3>::N : PF<2>::N);
     HIP_TRY(hipStreamSynchronize(h->stream));
     {
-        // unsort the planes -> second buffer, act -> act2, pid -> identity
         const int gp = (int)ceil_div(np, 256);
         if (f32)
             hipLaunchKernelGGL(k_unsort<uint32_t>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->Pw, np, h->Pw2, h->pid);
         else
             hipLaunchKernelGGL(k_unsort<double>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->P, np, h->P2, h->pid);
-        int *tmp_act = nullptr;
-        HIP_TRY(hipMalloc(&tmp_act, sizeof(int) * np));
-        hipLaunchKernelGGL(k_unsort_act, dim3(gp), dim3(256), 0, h->stream, np, h->act, tmp_act, h->pid);
-        std::vector<int> acts(np);
-        HIP_TRY(hipMemcpyAsync(acts.data(), tmp_act, sizeof(int) * np, hipMemcpyDeviceToHost, h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        HIP_TRY(hipFree(tmp_act));
-        std::vector<int8_t> a8(np);
-        for (long long i = 0; i < np; i++) a8[i] = (int8_t)acts[i];
-        for (long long i = 0; i < n; i++) a8[B.p_off + i] = active[i] ? 1 : 0;
-        HIP_TRY(hipMemcpyAsync(h->act2, a8.data(), np, hipMemcpyHostToDevice, h->stream));
-        std::vector<int> ids(np);
-        for (long long i = 0; i < np; i++) ids[i] = (int)i;
-        HIP_TRY(hipMemcpyAsync(h->pid2, ids.data(), sizeof(int) * np, hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL(k_unsort_act8, dim3(gp), dim3(256), 0, h->stream, np, h->act, h->act2, h->pid);
+        hipLaunchKernelGGL(k_iota, dim3(gp), dim3(256), 0, h->stream, h->pid2, np);
+        HIP_TRY(hipGetLastError());
+        if (B.n > n) HIP_TRY(hipMemsetAsync(h->act2 + B.p_off + n, 0, B.n - n, h->stream));  // unused capacity
+        if (n > 0) {
+            std::vector<int8_t> a8(n);
+            for (long long i = 0; i < n; i++) a8[i] = active[i] ? 1 : 0;
+            HIP_TRY(hipMemcpyAsync(h->act2 + B.p_off, a8.data(), n, hipMemcpyHostToDevice, h->stream));
+            HIP_TRY(hipStreamSynchronize(h->stream));
+        }
         const int NTS = dim == 3 ? 6 : 4;
         if (f32) {
             // precision form: cell + 32-bit fixed-point offset (x = h (cell + q 2^-32)),
@@ -639,9 +788,7 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
                 HIP_TRY(hipMemcpyAsync(h->Pw2 + (long long)f * np + B.p_off, pl.data() + (size_t)f * n,
                                        sizeof(uint32_t) * n, hipMemcpyHostToDevice, h->stream));
             HIP_TRY(hipStreamSynchronize(h->stream));
-            std::swap(h->Pw, h->Pw2);
-            std::swap(h->act, h->act2);
-            std::swap(h->pid, h->pid2);
+            swap_buffers(h);
             h->x0[body].assign(x, x + (long long)dim * n);
             h->uploaded[body] = 1;
             h->sorted = false;
@@ -674,13 +821,114 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
         HIP_TRY(put(FMASS, m));
         HIP_TRY(put(FEXT, extent));
         HIP_TRY(hipStreamSynchronize(h->stream));
-        std::swap(h->P, h->P2);
-        std::swap(h->act, h->act2);
-        std::swap(h->pid, h->pid2);
+        swap_buffers(h);
     }
     h->x0[body].assign(x, x + (long long)dim * n);
     h->uploaded[body] = 1;
     h->sorted = false;
+    return MPM_OK;
+}
+
+static int deactivate_body(mpm_gpu_handle *h, int b) {
+    const BodyDev &B = h->lo.bodies[b];
+    if (B.n == 0) return MPM_OK;
+    const long long np = h->lo.np;
+    hipLaunchKernelGGL(k_deactivate, dim3((unsigned)ceil_div(np, 256)), dim3(256), 0, h->stream, np, h->act, h->pid,
+                       B.p_off, B.p_off + B.n);
+    HIP_TRY(hipGetLastError());
+    return MPM_OK;
+}
+
+int mpm_gpu_load_sim(mpm_gpu_handle *h, int sim, const mpm_sim_desc *d) {
+    if (!h || !d) return fail(MPM_ERR_ARG, "handle and descriptor are required");
+    if (sim < 0 || sim >= (int)h->lo.sims.size()) return fail(MPM_ERR_ARG, "sim %d out of range", sim);
+    SimDev &s = h->lo.sims[sim];
+    const mpm_grid_desc &g = d->grid;
+    if (g.dim != h->lo.dim) return fail(MPM_ERR_ARG, "dim must match the handle (%d)", h->lo.dim);
+    if (d->nbody < 1 || d->nbody > s.nbody) return fail(MPM_ERR_ARG, "sim %d holds 1..%d bodies", sim, s.nbody);
+    if (d->ncontact < 0 || d->ncontact > h->contact_cap[sim])
+        return fail(MPM_ERR_ARG, "sim %d holds at most %d contacts", sim, h->contact_cap[sim]);
+    if (g.cpdi_spec != 0 && g.cpdi_spec != 1) return fail(MPM_ERR_ARG, "Unrecognized Argument for use_cpdi: %d", g.cpdi_spec);
+    if (g.contact_spec != 0 && g.contact_spec != 1)
+        return fail(MPM_ERR_ARG, "ERROR: Unknown SPEC in contact: %d", g.contact_spec);
+    for (int k = 0; k < g.dim; k++)
+        if (!(g.L[k] > 0) || g.N[k] < 1 || g.N[k] > h->grid_cap[sim][k])
+            return fail(MPM_ERR_ARG, "sim %d: need L > 0 and 1 <= N <= %d in dimension %d", sim, h->grid_cap[sim][k], k);
+    for (int b = 0; b < d->nbody; b++)
+        if (d->bodies[b].n_points < 0 || d->bodies[b].n_points > h->lo.bodies[s.body0 + b].n)
+            return fail(MPM_ERR_ARG, "sim %d body %d: at most %lld points", sim, b, (long long)h->lo.bodies[s.body0 + b].n);
+    for (int c = 0; c < d->ncontact; c++) {
+        const mpm_contact_desc &cd = d->contacts[c];
+        if (cd.body1 < 0 || cd.body1 >= d->nbody || cd.body2 < 0 || cd.body2 >= d->nbody || cd.body1 == cd.body2)
+            return fail(MPM_ERR_ARG, "contact %d: bodies must be two distinct sim-local indices", c);
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    // grid geometry; the node and particle-block storage keeps its creation size
+    s.element_count = 1;
+    for (int k = 0; k < g.dim; k++) {
+        s.N[k] = g.N[k];
+        s.L[k] = g.L[k];
+        s.h[k] = g.L[k] / g.N[k];
+        s.invh[k] = 1.0 / s.h[k];
+        s.NoverL[k] = (double)g.N[k] / g.L[k];
+        s.ftol[k] = std::ldexp((double)g.N[k], -48);
+        s.hf[k] = (float)s.h[k];
+        s.invhf[k] = (float)s.invh[k];
+        s.element_count *= g.N[k];
+    }
+    for (int k = 0; k < 3; k++) s.g[k] = k < g.dim ? g.gravity[k] : 0.0;
+    s.cpdi_spec = g.cpdi_spec;
+    s.contact_spec = g.contact_spec;
+    s.ncontact = d->ncontact;
+    for (int b = 0; b < s.nbody; b++) {
+        const int gb = s.body0 + b;
+        BodyDev &B = h->lo.bodies[gb];
+        int rc = deactivate_body(h, gb);
+        if (rc) return rc;
+        B.is_b1 = 0;
+        if (b < d->nbody) {
+            const mpm_body_desc &bd = d->bodies[b];
+            B.has_material = bd.has_material;
+            B.has_boundary = bd.has_boundary;
+            const double G = bd.E / (2.0 * (1.0 + bd.nu));
+            const double K = bd.E / (3.0 * (1.0 - 2 * bd.nu));
+            B.twoG = bd.has_material ? 2 * G : 0.0;
+            B.lambda = bd.has_material ? K - 2.0 * G / 3.0 : 0.0;
+            h->count[gb] = bd.n_points;
+        } else {
+            h->count[gb] = 0;
+        }
+        h->uploaded[gb] = h->count[gb] == 0;
+        h->x0[gb].clear();
+    }
+    for (int c = 0; c < d->ncontact; c++) {
+        h->lo.contacts[s.contact0 + c] = ContactDev{d->contacts[c].body1, d->contacts[c].body2, d->contacts[c].mu};
+        h->lo.bodies[s.body0 + d->contacts[c].body1].is_b1 = 1;
+    }
+    HIP_TRY(hipMemcpyAsync(h->d_bodies, h->lo.bodies.data(), sizeof(BodyDev) * h->lo.bodies.size(),
+                           hipMemcpyHostToDevice, h->stream));
+    if (!h->lo.contacts.empty())
+        HIP_TRY(hipMemcpyAsync(h->d_contacts, h->lo.contacts.data(), sizeof(ContactDev) * h->lo.contacts.size(),
+                               hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->sims_dirty = true;
+    return MPM_OK;
+}
+
+int mpm_gpu_clear_sim(mpm_gpu_handle *h, int sim) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    if (sim < 0 || sim >= (int)h->lo.sims.size()) return fail(MPM_ERR_ARG, "sim %d out of range", sim);
+    SimDev &s = h->lo.sims[sim];
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int b = 0; b < s.nbody; b++) {
+        int rc = deactivate_body(h, s.body0 + b);
+        if (rc) return rc;
+        h->count[s.body0 + b] = 0;
+        h->uploaded[s.body0 + b] = 1;
+    }
+    s.ncontact = 0;
+    h->sims_dirty = true;
+    HIP_TRY(hipStreamSynchronize(h->stream));
     return MPM_OK;
 }
 
@@ -701,24 +949,67 @@ int mpm_gpu_set_option(mpm_gpu_handle *h, int key, int value) {
         case MPM_OPT_FUSED_PASS:
             h->fused = value != 0;
             return MPM_OK;
+        case MPM_OPT_GRAPH:
+            h->use_graph = value != 0;
+            return MPM_OK;
+        case MPM_OPT_AUTOTUNE:
+            h->autotune = value != 0;
+            h->tuned = false;
+            return MPM_OK;
+        case MPM_OPT_PASS_GRID:
+        case MPM_OPT_GRID_WGS:
+            if (value < 1 || value > (1 << 20)) return fail(MPM_ERR_ARG, "workgroup count out of range");
+            (key == MPM_OPT_PASS_GRID ? h->pass_grid : h->grid_wgs) = value;
+            return MPM_OK;
         default:
             return fail(MPM_ERR_ARG, "unknown option %d", key);
     }
 }
 
+int mpm_gpu_get_option(mpm_gpu_handle *h, int key, int *value) {
+    if (!h || !value) return fail(MPM_ERR_ARG, "handle and value are required");
+    switch (key) {
+        case MPM_OPT_SORT_INTERVAL: *value = h->sort_interval; return MPM_OK;
+        case MPM_OPT_FUSED_PASS: *value = h->fused; return MPM_OK;
+        case MPM_OPT_GRAPH: *value = h->use_graph; return MPM_OK;
+        case MPM_OPT_AUTOTUNE: *value = h->autotune ? (h->tuned ? 2 : 1) : 0; return MPM_OK;
+        case MPM_OPT_PASS_GRID: *value = h->pass_grid; return MPM_OK;
+        case MPM_OPT_GRID_WGS: *value = h->grid_wgs; return MPM_OK;
+        case MPM_OPT_GRAPH_CAPTURES: *value = h->graph_captures; return MPM_OK;
+        default: return fail(MPM_ERR_ARG, "unknown option %d", key);
+    }
+}
+
+static int run_steps(mpm_gpu_handle *h, const double *dt, int nsim, int nsteps);
+
 int mpm_gpu_step(mpm_gpu_handle *h, double dt, int nsteps) {
     if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    std::vector<double> dts(h->lo.sims.size(), dt);
+    return run_steps(h, dts.data(), (int)dts.size(), nsteps);
+}
+
+int mpm_gpu_step_batch(mpm_gpu_handle *h, const double *dt, int nsim, int nsteps) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    if (!dt) return fail(MPM_ERR_ARG, "dt is NULL");
+    return run_steps(h, dt, nsim, nsteps);
+}
+
+static int run_steps(mpm_gpu_handle *h, const double *dt, int nsim, int nsteps) {
     if (nsteps < 0) return fail(MPM_ERR_ARG, "nsteps must be >= 0");
-    if (!(dt > 0) || !std::isfinite(dt)) return fail(MPM_ERR_ARG, "dt must be positive and finite");
+    if (nsim != (int)h->lo.sims.size()) return fail(MPM_ERR_ARG, "need %zu time steps, got %d", h->lo.sims.size(), nsim);
+    for (int i = 0; i < nsim; i++)
+        if (!(dt[i] > 0) || !std::isfinite(dt[i])) return fail(MPM_ERR_ARG, "dt must be positive and finite");
     for (size_t b = 0; b < h->uploaded.size(); b++)
-        if (!h->uploaded[b] && h->lo.bodies[b].n > 0) return fail(MPM_ERR_STATE, "body %zu has not been uploaded", b);
+        if (!h->uploaded[b] && h->count[b] > 0) return fail(MPM_ERR_STATE, "body %zu has not been uploaded", b);
     if (nsteps == 0) return MPM_OK;
     bool changed = h->sims_dirty;
-    for (SimDev &s : h->lo.sims)
-        if (s.dt != dt) {
-            s.dt = dt;
+    for (int i = 0; i < nsim; i++) {
+        SimDev &s = h->lo.sims[i];
+        if (s.dt != dt[i]) {
+            s.dt = dt[i];
             changed = true;
         }
+    }
     if (changed) {
         HIP_TRY(hipMemcpyAsync(h->d_sims, h->lo.sims.data(), sizeof(SimDev) * h->lo.sims.size(), hipMemcpyHostToDevice,
                                h->stream));
@@ -755,7 +1046,7 @@ int mpm_gpu_download(mpm_gpu_handle *h, int body, double *x, double *v, double *
     if (body < 0 || body >= (int)h->lo.bodies.size()) return fail(MPM_ERR_ARG, "body %d out of range", body);
     const BodyDev &B = h->lo.bodies[body];
     const int dim = h->lo.dim;
-    const long long n = B.n, np = h->lo.np;
+    const long long n = h->count[body], np = h->lo.np;
     if (n == 0) return MPM_OK;
     const bool f32 = h->precision == MPM_PREC_FP32;
     const int nf = f32 ? (dim == 3 ? PW<3>::N : PW<2>::N) : (dim == 3 ? PF<3>::N : PF<2>::N);
@@ -826,7 +1117,7 @@ int mpm_gpu_download_aux(mpm_gpu_handle *h, int body, double *u, double *L) {
     if (body < 0 || body >= (int)h->lo.bodies.size()) return fail(MPM_ERR_ARG, "body %d out of range", body);
     const BodyDev &B = h->lo.bodies[body];
     const int dim = h->lo.dim;
-    const long long n = B.n, np = h->lo.np;
+    const long long n = h->count[body], np = h->lo.np;
     if (n == 0) return MPM_OK;
     if (u) {
         int rc = mpm_gpu_download(h, body, u, nullptr, nullptr, nullptr, nullptr);
@@ -859,7 +1150,7 @@ int mpm_gpu_connectivity(mpm_gpu_handle *h, int body, int32_t *corner_elem, uint
     const BodyDev &B = h->lo.bodies[body];
     const int dim = h->lo.dim;
     const int nc = 1 << dim;
-    const long long n = B.n, np = h->lo.np;
+    const long long n = h->count[body], np = h->lo.np;
     if (n == 0) return MPM_OK;
     int *d_elem = nullptr;
     uint8_t *d_code = nullptr;
@@ -1030,6 +1321,8 @@ int mpm_gpu_destroy(mpm_gpu_handle *h) {
         if (p) (void)hipFree(p);
     for (unsigned *p : h->scan_tmp)
         if (p) (void)hipFree(p);
+    for (hipGraphExec_t g : h->graph)
+        if (g) (void)hipGraphExecDestroy(g);
     for (Timer &t : h->free_timers) {
         (void)hipEventDestroy(t.a);
         (void)hipEventDestroy(t.b);

@@ -40,8 +40,8 @@ __device__ void mark_range(const KArgs &A, const SimDev &s, const int *lo, const
     int b0[3] = {0, 0, 0}, b1[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        b0[d] = (lo[d] + 1) >> L;  // node storage is shifted by one (node_addr)
-        b1[d] = (hi[d] + 1) >> L;
+        b0[d] = (lo[d] + NSHIFT) >> L;  // node storage is shifted (node_addr)
+        b1[d] = (hi[d] + NSHIFT) >> L;
     }
     for (int k = b0[2]; k <= b1[2]; k++)
         for (int j = b0[1]; j <= b1[1]; j++)
@@ -535,14 +535,14 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
             long long t = lb;
             const int bx = (int)(t % s.NBn[0]);
             t /= s.NBn[0];
-            ijk[0] = (bx << L) + (lane & (NB - 1)) - 1;  // storage shifted by one node
+            ijk[0] = (bx << L) + (lane & (NB - 1)) - NSHIFT;  // storage is shifted (node_addr)
             if (DIM == 3) {
                 const int by = (int)(t % s.NBn[1]);
                 const int bz = (int)(t / s.NBn[1]);
-                ijk[1] = (by << L) + ((lane >> L) & (NB - 1)) - 1;
-                ijk[2] = (bz << L) + (lane >> (2 * L)) - 1;
+                ijk[1] = (by << L) + ((lane >> L) & (NB - 1)) - NSHIFT;
+                ijk[2] = (bz << L) + (lane >> (2 * L)) - NSHIFT;
             } else {
-                ijk[1] = ((int)t << L) + (lane >> L) - 1;
+                ijk[1] = ((int)t << L) + (lane >> L) - NSHIFT;
             }
         }
         bool valid = true;

@@ -144,7 +144,7 @@ struct Prec32 {
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
             q.q[d] = A.Pw[(F::Q + d) * A.np + p];
-            q.r[d] = (float)q.q[d] * 0x1p-32f;
+            q.r[d] = offset(q.q[d]);
             q.v[d] = Pf[(F::V + d) * A.np + p];
         }
 #pragma unroll
@@ -171,6 +171,9 @@ struct Prec32 {
         for (int c = 0; c < F::NTS; c++) Pf[(F::T + c) * A.np + p] = q.T6[c];
         Pf[F::LNJ * A.np + p] = q.lnJ;
     }
+
+    // q 2^-32 in FP32, rounded toward zero so that r < 1 and the point stays in cell c
+    __device__ __forceinline__ static float offset(uint32_t q) { return __uint2float_rz(q) * 0x1p-32f; }
 
     // x = h (c + r) is inside [0, L] in dimension d: element E = c + floor(r) with
     // 0 <= E < N, or exactly on the last node (E == N, frac 0). NaN is outside.
@@ -283,16 +286,20 @@ struct Prec32 {
                     }
                 } else {
                     S = tmp;
+                    // cartesian.cpp:307-357 computes -N / (1 - |r_d|) * r_d / |r_d| / h_d and
+                    // zeroes it at |r_d| >= 1. For the element's own nodes that equals the
+                    // product form below except exactly on a node line (r_d = 0: 0/0;
+                    // |r_d| = 1: zero instead of the limit). In FP64 a point lands there with
+                    // probability ~2^-52 per step; on the 2^-32 fixed-point grid of this form
+                    // it happens routinely, so the form takes the one-sided derivative of
+                    // the element the point is in -- what FP64 computes a hair off the line.
 #pragma unroll
                     for (int d = 0; d < DIM; d++) {
-                        // cartesian.cpp:325-333, including 0/0 = NaN on a node line
-                        float gi = -tmp;
-                        gi = gi / (1.0f - fabsf(rn[d]));
-                        gi = gi * rn[d];
-                        gi = gi / fabsf(rn[d]);
-                        gi = gi / s.hf[d];
-                        if (fabsf(rn[d]) >= 1.0f) gi = 0.0f;
-                        G[d] = gi;
+                        float others = 1.0f;
+#pragma unroll
+                        for (int e = 0; e < DIM; e++)
+                            if (e != d) others *= (1.0f - fabsf(rn[e]));
+                        G[d] = -copysignf(1.0f, rn[d]) * others / s.hf[d];
                     }
                 }
                 fn(nijk, S, G);
@@ -331,11 +338,13 @@ struct Prec32 {
                 q.c[d] = -1;
                 q.q[d] = 0u;
             }
+            q.r[d] = offset(q.q[d]);  // the fused pass maps the moved point
             q.v[d] = q.v[d] + dvp[d];
         }
         float tr = Lp[0][0] + Lp[1][1];
         if (DIM == 3) tr = tr + Lp[DIM - 1][DIM - 1];
         q.lnJ = q.lnJ + dtf * tr;
+        q.V = q.V * expf(dtf * tr);  // the fused pass scatters with the updated volume
         if (B.has_material) {
             float L3[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
@@ -356,7 +365,7 @@ struct Prec32 {
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
             qq[d] = A.Pw[(F::Q + d) * A.np + p];
-            in = in && inside(c[d], (float)qq[d] * 0x1p-32f, s.N[d]);
+            in = in && inside(c[d], offset(qq[d]), s.N[d]);
         }
         if (!in) return false;
         oct = 0;

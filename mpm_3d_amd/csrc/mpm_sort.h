@@ -175,6 +175,27 @@ __global__ __launch_bounds__(256) void k_unsort(long long np, int nf, const E *s
     for (int f = 0; f < nf; f++) dst[f * np + o] = src[f * src_stride + p];
 }
 
+__global__ __launch_bounds__(256) void k_unsort_act8(long long np, const int8_t *act, int8_t *dst, const int *pid) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int o = pid[p];
+    if (o >= 0) dst[o] = act[p];
+}
+
+__global__ __launch_bounds__(256) void k_iota(int *pid, long long np) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < np) pid[p] = (int)p;
+}
+
+// deactivate the points whose original index lies in [lo, hi)
+__global__ __launch_bounds__(256) void k_deactivate(long long np, int8_t *act, const int *pid, long long lo,
+                                                    long long hi) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int o = pid[p];
+    if (o >= lo && o < hi) act[p] = 0;
+}
+
 __global__ __launch_bounds__(256) void k_unsort_act(long long np, const int8_t *act, int *dst, const int *pid) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;

@@ -27,10 +27,15 @@ SYMBOLS = (
     "mpm_gpu_download_aux", "mpm_gpu_connectivity", "mpm_gpu_set_sort_interval", "mpm_gpu_set_timing",
     "mpm_gpu_get_timing", "mpm_gpu_stats", "mpm_gpu_synchronize", "mpm_gpu_destroy",
     "mpm_gpu_last_error", "mpm_gpu_footprint", "mpm_gpu_host_weights", "mpm_gpu_copy_probe",
-    "mpm_gpu_set_option",
+    "mpm_gpu_set_option", "mpm_gpu_get_option", "mpm_gpu_step_batch", "mpm_gpu_load_sim", "mpm_gpu_clear_sim",
 )
 MPM_OPT_SORT_INTERVAL = 1
 MPM_OPT_FUSED_PASS = 2
+MPM_OPT_GRAPH = 3
+MPM_OPT_AUTOTUNE = 4
+MPM_OPT_PASS_GRID = 5
+MPM_OPT_GRID_WGS = 6
+MPM_OPT_GRAPH_CAPTURES = 7
 
 
 class MpmGpuError(RuntimeError):
@@ -80,6 +85,10 @@ def load_library(path: str = LIB_PATH):
     lib.mpm_gpu_set_sort_interval.argtypes = [vp, C.c_int]
     lib.mpm_gpu_set_timing.argtypes = [vp, C.c_int]
     lib.mpm_gpu_set_option.argtypes = [vp, C.c_int, C.c_int]
+    lib.mpm_gpu_get_option.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    lib.mpm_gpu_step_batch.argtypes = [vp, _dp, C.c_int, C.c_int]
+    lib.mpm_gpu_load_sim.argtypes = [vp, C.c_int, C.POINTER(SimDesc)]
+    lib.mpm_gpu_clear_sim.argtypes = [vp, C.c_int]
     lib.mpm_gpu_get_timing.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     lib.mpm_gpu_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.mpm_gpu_synchronize.argtypes = [vp]
@@ -121,10 +130,32 @@ def _grid_desc(dim, L, N, cpdi_spec, contact_spec, gravity):
     return g
 
 
+def _sim_desc(w, keep):
+    """SimDesc of a workload; `keep` holds the ctypes arrays it points to."""
+    bds = (BodyDesc * len(w.bodies))()
+    for bi, b in enumerate(w.bodies):
+        bds[bi].n_points = b.n
+        bds[bi].has_material = int(b.E is not None)
+        bds[bi].E = float(b.E or 0.0)
+        bds[bi].nu = float(b.nu or 0.0)
+        bds[bi].has_boundary = int(b.boundary)
+    cds = (ContactDesc * max(1, len(w.contacts)))()
+    for ci, c in enumerate(w.contacts):
+        cds[ci].body1, cds[ci].body2, cds[ci].mu = c.body1, c.body2, c.mu
+    keep += [bds, cds]
+    d = SimDesc()
+    d.grid = _grid_desc(w.dim, w.L, w.N, w.cpdi_spec, w.contact_spec, w.gravity)
+    d.nbody = len(w.bodies)
+    d.bodies = bds
+    d.ncontact = len(w.contacts)
+    d.contacts = cds
+    return d
+
+
 class GpuSim:
     """One handle on the GPU holding one or more simulations (``workloads``)."""
 
-    def __init__(self, workloads, precision=MPM_PREC_FP64, sort_interval=16, fused=False):
+    def __init__(self, workloads, precision=MPM_PREC_FP64, sort_interval=16, fused=False, upload=True):
         if not isinstance(workloads, (list, tuple)):
             workloads = [workloads]
         self.lib = load_library()
@@ -133,38 +164,55 @@ class GpuSim:
         keep = []
         sims = (SimDesc * len(self.workloads))()
         self.body_index = []  # (sim, local) per global body
+        self.body0 = []       # first global body of each sim
         for si, w in enumerate(self.workloads):
-            bds = (BodyDesc * len(w.bodies))()
-            for bi, b in enumerate(w.bodies):
-                bds[bi].n_points = b.n
-                bds[bi].has_material = int(b.E is not None)
-                bds[bi].E = float(b.E or 0.0)
-                bds[bi].nu = float(b.nu or 0.0)
-                bds[bi].has_boundary = int(b.boundary)
+            self.body0.append(len(self.body_index))
+            sims[si] = _sim_desc(w, keep)
+            for bi in range(len(w.bodies)):
                 self.body_index.append((si, bi))
-            cds = (ContactDesc * max(1, len(w.contacts)))()
-            for ci, c in enumerate(w.contacts):
-                cds[ci].body1, cds[ci].body2, cds[ci].mu = c.body1, c.body2, c.mu
-            keep += [bds, cds]
-            sims[si].grid = _grid_desc(w.dim, w.L, w.N, w.cpdi_spec, w.contact_spec, w.gravity)
-            sims[si].nbody = len(w.bodies)
-            sims[si].bodies = bds
-            sims[si].ncontact = len(w.contacts)
-            sims[si].contacts = cds
         self._keep = keep
         self._sims = sims
+        self.current = [self.workloads[si].bodies[bi] for si, bi in self.body_index]
         h = C.c_void_p()
         _check(self.lib.mpm_gpu_create_batch(sims, len(self.workloads), precision, C.byref(h)))
         self.h = h
         _check(self.lib.mpm_gpu_set_sort_interval(self.h, sort_interval))
         _check(self.lib.mpm_gpu_set_option(self.h, MPM_OPT_FUSED_PASS, int(bool(fused))))
-        for gb, (si, bi) in enumerate(self.body_index):
-            self.upload(gb, self.workloads[si].bodies[bi])
+        if upload:
+            for gb, (si, bi) in enumerate(self.body_index):
+                self.upload(gb, self.workloads[si].bodies[bi])
+        else:  # capacity only: every slot starts empty
+            for si in range(len(self.workloads)):
+                self.clear_sim(si)
 
     # -- state transfer -------------------------------------------------------------
     def _body(self, gb):
-        si, bi = self.body_index[gb]
-        return self.workloads[si].bodies[bi]
+        return self.current[gb]
+
+    def load_sim(self, si, w):
+        """Reconfigure slot si for workload w (within the slot's creation sizes) and upload it."""
+        keep = []
+        d = _sim_desc(w, keep)
+        _check(self.lib.mpm_gpu_load_sim(self.h, si, C.byref(d)))
+        b0 = self.body0[si]
+        for bi, b in enumerate(w.bodies):
+            self.current[b0 + bi] = b
+            self.upload(b0 + bi, b)
+
+    def clear_sim(self, si):
+        _check(self.lib.mpm_gpu_clear_sim(self.h, si))
+
+    def step_batch(self, dts, nsteps):
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        _check(self.lib.mpm_gpu_step_batch(self.h, d.ctypes.data_as(_dp), len(d), int(nsteps)))
+
+    def set_option(self, key, value):
+        _check(self.lib.mpm_gpu_set_option(self.h, key, int(value)))
+
+    def get_option(self, key):
+        v = C.c_int()
+        _check(self.lib.mpm_gpu_get_option(self.h, key, C.byref(v)))
+        return v.value
 
     def upload(self, gb, body, T=None):
         dim = self.dim

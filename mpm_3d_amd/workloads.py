@@ -203,3 +203,83 @@ def write_points(path, m, v, x, xt, active):
 def example_dir():
     return os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests",
                         "golden", "examples")
+
+
+def wave_speed(E, nu, rho):
+    """P-wave speed sqrt((K + 4G/3) / rho) of IsotropicLinearElasticity."""
+    G = E / (2 * (1 + nu))
+    K = E / (3 * (1 - 2 * nu))
+    return float(np.sqrt((K + 4 * G / 3) / rho))
+
+
+def ensemble_2d(n_sims=512, seed=0xC0FFEE + 3, steps=2000, rho=1000.0):
+    """Batched 2D ensemble: independent block drops drawn from `seed`. Grid N x N on a
+    1 m square with N uniform in 16..256; points log-uniform in 1e3..2e5 at 2x2 per cell,
+    clamped to half the domain's cells; E log-uniform in 1e4..1e7 Pa, nu uniform in
+    0.2..0.4, drop height (block bottom) uniform in 0.1..0.5 m; dt = 0.2 h / c."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_sims):
+        N = int(rng.integers(16, 257))
+        h = 1.0 / N
+        npts = float(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
+        npts = min(npts, 4 * N * N * 0.5)
+        side = max(2, int(round(np.sqrt(npts / 4))))  # cells per side (2x2 points per cell)
+        side = min(side, N - 4)
+        E = float(np.exp(rng.uniform(np.log(1e4), np.log(1e7))))
+        nu = float(rng.uniform(0.2, 0.4))
+        y0 = float(rng.uniform(0.1, 0.5))
+        c0 = (N - side) // 2
+        j0 = int(round(y0 / h))
+        j0 = max(2, min(j0, N - side - 2))
+        lo = (c0 * h, j0 * h)
+        hi = ((c0 + side) * h, (j0 + side) * h)
+        m, v, x, xt = lattice_block(2, np.array([h, h]), lo, hi, 2, rho)
+        body = Body("block", m, v, x, xt, np.ones(len(m), np.int32), E, nu, True)
+        dt = 0.2 * h / wave_speed(E, nu, rho)
+        out.append(Workload(f"ens{i}_N{N}_{len(m)}pts", 2, (1.0, 1.0), (N, N), [body], [], dt, steps, (0.0, -9.81)))
+    return out
+
+
+def serve_requests(n, seed=0xC0FFEE + 4, rho=1000.0):
+    """Served 3D stream: block drops or two-block collisions drawn from `seed`. Grid N^3
+    on a 1 m cube with N uniform in 32..128, 2^3 points per cell, 1e4..2e6 points, E
+    log-uniform in 1e5..1e8 Pa, nu = 0.3, steps uniform in 500..5000, dt = 0.2 h / c.
+    Returns (workload, steps) pairs."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        N = int(rng.integers(32, 129))
+        h = 1.0 / N
+        E = float(np.exp(rng.uniform(np.log(1e5), np.log(1e8))))
+        nu = 0.3
+        steps = int(rng.integers(500, 5001))
+        collide = bool(rng.integers(0, 2))
+        dt = 0.2 * h / wave_speed(E, nu, rho)
+        hv = np.array([h, h, h])
+        if not collide:
+            s_max = min(N // 2 - 2, 62)
+            s = int(rng.integers(11, s_max + 1))  # cells per side: 8 s^3 points
+            c0 = (N - s) // 2
+            k0 = int(rng.integers(2, max(3, N - s - 2)))
+            lo = (c0 * h, c0 * h, k0 * h)
+            hi = ((c0 + s) * h, (c0 + s) * h, (k0 + s) * h)
+            m, v, x, xt = lattice_block(3, hv, lo, hi, 2, rho)
+            bodies = [Body("block", m, v, x, xt, np.ones(len(m), np.int32), E, nu, True)]
+            contacts = []
+        else:
+            s_max = min((N - 6) // 2 - 1, 49)
+            s = int(rng.integers(9, max(10, s_max + 1)))
+            c0 = (N - 2 * s - 2) // 2
+            cy = (N - s) // 2
+            bodies = []
+            for b, (i0, vx) in enumerate(((c0, 1.0), (c0 + s + 2, -1.0))):
+                lo = (i0 * h, cy * h, cy * h)
+                hi = ((i0 + s) * h, (cy + s) * h, (cy + s) * h)
+                m, v, x, xt = lattice_block(3, hv, lo, hi, 2, rho, velocity=(vx, 0, 0))
+                bodies.append(Body(f"block{b}", m, v, x, xt, np.ones(len(m), np.int32), E, nu, True))
+            contacts = [Contact(0, 1, 0.4)]
+        w = Workload(f"req{i}_N{N}_{'collide' if collide else 'drop'}", 3, (1.0, 1.0, 1.0), (N, N, N), bodies,
+                     contacts, dt, steps, (0.0, 0.0, -9.81))
+        out.append((w, steps))
+    return out

@@ -125,3 +125,48 @@ def test_precision_column_1000_steps(size):
     assert e["x_cells"] <= 1e-5, e
     assert e["v_rel"] <= 1e-5 and e["T_rel"] <= 1e-4 and e["V_rel"] <= 1e-5, e
     assert e["energy_rel"] <= 1e-4, e
+
+
+def test_precision_form_points_on_node_lines_stay_finite():
+    """CPDI-off points (within a corner offset of the floor) sitting exactly on node lines
+    of the fixed-point grid: the FP32 form takes the one-sided gradient there, so nothing
+    turns NaN, and it behaves as the same form does 1e-6 h off the lines (4295 units of
+    the 2^-32 grid, where the reference's gradient is finite): against the FP64 form run
+    off the lines, its distance is within twice the FP32-off-line distance plus the 1e-6 h
+    shift's own effect. Exactly on the lines the FP64 form reproduces the reference's 0/0."""
+    h = 1.0 / 20
+    xs = np.array([i * h for i in range(4, 16)])
+    x = np.stack([np.repeat(xs, 3), np.tile([0.001, 0.002, 0.03], len(xs))], axis=1)
+    n = len(x)
+    V = np.full(n, (h / 2) ** 2)
+
+    def wl(xx):
+        body = W.Body("row", 1000.0 * V, V, xx, np.zeros((n, 2)), np.ones(n, np.int32), 1e5, 0.3, True)
+        return W.Workload("nodeline", 2, (1.0, 1.0), (20, 20), [body], [], 1e-4, 50, (0.0, -9.81))
+
+    on, off = wl(x), wl(x + np.array([1e-6 * h, 0.0]))
+    ref = gpu_of(off, precision=MPM_PREC_FP64)
+    f_on = gpu_of(on, precision=MPM_PREC_FP32)
+    f_off = gpu_of(off, precision=MPM_PREC_FP32)
+    for g in (ref, f_on, f_off):
+        g.step(1e-4, 50)
+    sr, son, soff = _gpu_state(ref, 0), _gpu_state(f_on, 0), _gpu_state(f_off, 0)
+    assert np.all(np.isfinite(son["xt"])) and np.all(son["active"] == 1)
+    e_on, e_off = _errors(on, sr, son, 50e-4), _errors(off, sr, soff, 50e-4)
+    print(f"node lines: on {e_on} off {e_off}")
+    for k in ("x_cells", "v_rel", "T_rel"):
+        assert e_on[k] <= 2 * e_off[k] + 1e-5, k
+
+
+def test_precision_fused_matches_split():
+    """The fused G2P+P2G pass of the precision form gives the split passes' state to FP32
+    rounding (it scatters with the volume updated in the same pass)."""
+    w = example_workload("collision3D")
+    a = gpu_of(w, precision=MPM_PREC_FP32, fused=True)
+    b = gpu_of(w, precision=MPM_PREC_FP32, fused=False)
+    a.step(w.dt, 40)
+    b.step(w.dt, 40)
+    for bi in range(2):
+        e = _errors(w, _gpu_state(b, bi), _gpu_state(a, bi), 40 * w.dt)
+        print(f"fused vs split body {bi}: {e}")
+        assert e["x_cells"] <= 1e-6 and e["v_rel"] <= 1e-6 and e["T_rel"] <= 1e-6 and e["V_rel"] <= 1e-6
