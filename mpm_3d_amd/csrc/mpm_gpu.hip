@@ -202,6 +202,8 @@ struct mpm_gpu_handle {
     int *ncount = nullptr;
     unsigned *bcount = nullptr, *boff = nullptr;
     std::vector<unsigned *> scan_tmp;  // per level: block sums, their scan
+    unsigned *inact_count = nullptr, *inact_off = nullptr;  // inactive slots per sort workgroup
+    std::vector<unsigned *> scan_tmp_i;
     std::vector<long long> scan_n;
     PBlk *plist = nullptr;
     int *npblk = nullptr;
@@ -310,6 +312,30 @@ int dalloc(T **p, long long n) {
 
 void swap_buffers(mpm_gpu_handle *h);
 
+// exclusive prefix sum of n unsigned values, multi-level (1024 threads x 4 per block);
+// tmp holds two buffers per level beyond the first (alloc_scan_tmp)
+void scan_exclusive(mpm_gpu_handle *h, const unsigned *in, unsigned *outp, long long n,
+                    const std::vector<unsigned *> &tmp) {
+    std::vector<std::pair<unsigned *, long long>> stack;
+    int lvl = 0;
+    while (true) {
+        const long long nb = ceil_div(n, 4096);
+        unsigned *bsum = nb > 1 ? tmp[2 * lvl] : nullptr;
+        hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(1024), 0, h->stream, in, outp, bsum, n);
+        if (nb <= 1) break;
+        stack.push_back({outp, n});
+        in = bsum;
+        outp = tmp[2 * lvl + 1];
+        n = nb;
+        lvl++;
+    }
+    for (int l = (int)stack.size() - 1; l >= 0; l--) {
+        const long long nn = stack[l].second;
+        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)ceil_div(nn, 4096)), dim3(1024), 0, h->stream, stack[l].first,
+                           tmp[2 * l + 1], nn);
+    }
+}
+
 template <int DIM, class PT>
 int launch_sort(mpm_gpu_handle *h) {
     const long long np = h->lo.np, nk = h->lo.nkeys;
@@ -319,40 +345,20 @@ int launch_sort(mpm_gpu_handle *h) {
     if (rc) return rc;
     const int gp = (int)std::max(1ll, ceil_div(np, 256));
     hipLaunchKernelGGL((k_bin_count<DIM, PT>), dim3(gp), dim3(256), 0, h->stream, A, h->bcount, h->pkey, h->prank,
-                       nk - 1);
-    // exclusive scan of the histogram, multi-level
-    {
-        const unsigned *in = h->bcount;
-        unsigned *outp = h->boff;
-        long long n = nk;
-        std::vector<std::pair<unsigned *, long long>> stack;
-        int lvl = 0;
-        while (true) {
-            const long long nb = ceil_div(n, 4096);
-            unsigned *bsum = nb > 1 ? h->scan_tmp[2 * lvl] : nullptr;
-            hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(1024), 0, h->stream, in, outp, bsum, n);
-            if (nb <= 1) break;
-            stack.push_back({outp, n});
-            in = bsum;
-            outp = h->scan_tmp[2 * lvl + 1];
-            n = nb;
-            lvl++;
-        }
-        for (int l = (int)stack.size() - 1; l >= 0; l--) {
-            const long long nn = stack[l].second;
-            hipLaunchKernelGGL(k_scan_add, dim3((unsigned)ceil_div(nn, 4096)), dim3(1024), 0, h->stream,
-                               stack[l].first, h->scan_tmp[2 * l + 1], nn);
-        }
-    }
+                       nk - 1, h->inact_count);
+    scan_exclusive(h, h->bcount, h->boff, nk, h->scan_tmp);             // bucket offsets
+    scan_exclusive(h, h->inact_count, h->inact_off, gp, h->scan_tmp_i);  // inactive ranks per workgroup
     hipLaunchKernelGGL(k_make_plist, dim3((unsigned)ceil_div(nk, 256)), dim3(256), 0, h->stream, A, h->bcount,
                        h->boff, h->plist, h->npblk, nk, 1 << DIM);
     hipLaunchKernelGGL(k_clear, dim3((unsigned)ceil_div(nk, 256)), dim3(256), 0, h->stream, h->bcount, nk);
     if constexpr (PT::FP32) {
         hipLaunchKernelGGL((k_bin_scatter<uint32_t, PW<DIM>::N>), dim3(gp), dim3(256), 0, h->stream, np, h->Pw, h->Pw2,
-                           h->act, h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff);
+                           h->act, h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff, (int)(nk - 1),
+                           h->inact_off);
     } else {
         hipLaunchKernelGGL((k_bin_scatter<double, PF<DIM>::N>), dim3(gp), dim3(256), 0, h->stream, np, h->P, h->P2,
-                           h->act, h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff);
+                           h->act, h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff, (int)(nk - 1),
+                           h->inact_off);
     }
     swap_buffers(h);
     HIP_TRY(hipGetLastError());
@@ -631,15 +637,24 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     TRY(dalloc(&h->plist, L.npblk_total));
     TRY(dalloc(&h->npblk, 1));
     TRY(dalloc(&h->stats, 4));
-    for (long long n = L.nkeys; n > 4096;) {
-        const long long nb = ceil_div(n, 4096);
-        unsigned *a = nullptr, *b = nullptr;
-        TRY(dalloc(&a, nb));
-        TRY(dalloc(&b, nb));
-        h->scan_tmp.push_back(a);
-        h->scan_tmp.push_back(b);
-        n = nb;
-    }
+    auto alloc_scan_tmp = [&](long long n, std::vector<unsigned *> &tmp) -> int {
+        for (; n > 4096;) {
+            const long long nb = ceil_div(n, 4096);
+            unsigned *a = nullptr, *b = nullptr;
+            int r = dalloc(&a, nb);
+            if (!r) r = dalloc(&b, nb);
+            tmp.push_back(a);
+            tmp.push_back(b);
+            if (r) return r;
+            n = nb;
+        }
+        return MPM_OK;
+    };
+    TRY(alloc_scan_tmp(L.nkeys, h->scan_tmp));
+    const long long gp_sort = std::max(1ll, ceil_div(L.np, 256));
+    TRY(dalloc(&h->inact_count, gp_sort));
+    TRY(dalloc(&h->inact_off, gp_sort));
+    TRY(alloc_scan_tmp(gp_sort, h->scan_tmp_i));
     // zero state: accumulators must start at zero, flags clear, counters zero
     e = hipMemsetAsync(h->acc, 0, sizeof(double) * L.nt * nacc, h->stream);
     if (e == hipSuccess)
@@ -732,13 +747,25 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
     const int nf = f32 ? (dim == 3 ? PW<3>::N : PW<2>::N) : (dim == 3 ? PF<3>::N : PF<2>::N);
     HIP_TRY(hipStreamSynchronize(h->stream));
     {
+        // original order in the second buffer: planes, act and pid of every point in use;
+        // unused capacity stays act 0, pid -1
         const int gp = (int)ceil_div(np, 256);
+        HIP_TRY(hipMemsetAsync(h->act2, 0, np, h->stream));
+        hipLaunchKernelGGL(k_fill_i32, dim3(gp), dim3(256), 0, h->stream, h->pid2, 0ll, np, -1);
         if (f32)
-            hipLaunchKernelGGL(k_unsort<uint32_t>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->Pw, np, h->Pw2, h->pid);
+            hipLaunchKernelGGL(k_unsort<uint32_t>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->Pw, np, h->Pw2, h->pid,
+                               h->pid2);
         else
-            hipLaunchKernelGGL(k_unsort<double>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->P, np, h->P2, h->pid);
+            hipLaunchKernelGGL(k_unsort<double>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->P, np, h->P2, h->pid,
+                               h->pid2);
         hipLaunchKernelGGL(k_unsort_act8, dim3(gp), dim3(256), 0, h->stream, np, h->act, h->act2, h->pid);
-        hipLaunchKernelGGL(k_iota, dim3(gp), dim3(256), 0, h->stream, h->pid2, np);
+        // this body: its points get their identity, its unused capacity none
+        if (n > 0)
+            hipLaunchKernelGGL(k_iota_range, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, h->stream, h->pid2,
+                               B.p_off, B.p_off + n);
+        if (B.n > n)
+            hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)ceil_div(B.n - n, 256)), dim3(256), 0, h->stream, h->pid2,
+                               B.p_off + n, B.p_off + B.n, -1);
         HIP_TRY(hipGetLastError());
         if (B.n > n) HIP_TRY(hipMemsetAsync(h->act2 + B.p_off + n, 0, B.n - n, h->stream));  // unused capacity
         if (n > 0) {
@@ -1321,6 +1348,10 @@ int mpm_gpu_destroy(mpm_gpu_handle *h) {
         if (p) (void)hipFree(p);
     for (unsigned *p : h->scan_tmp)
         if (p) (void)hipFree(p);
+    for (unsigned *p : h->scan_tmp_i)
+        if (p) (void)hipFree(p);
+    if (h->inact_count) (void)hipFree(h->inact_count);
+    if (h->inact_off) (void)hipFree(h->inact_off);
     for (hipGraphExec_t g : h->graph)
         if (g) (void)hipGraphExecDestroy(g);
     for (Timer &t : h->free_timers) {

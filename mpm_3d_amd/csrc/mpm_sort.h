@@ -42,9 +42,14 @@ __device__ __forceinline__ int body_of_block(const KArgs &A, long long key) {
     return lo;
 }
 
+// Keys per slot and ranks inside their bucket. Active keys (a particle block and octant,
+// a few dozen points each) count with wavefront-aggregated atomics. The inactive bucket
+// can hold most of the slots (unused capacity of a slot pool), so it is ranked without
+// atomics: the rank inside the workgroup here, the workgroup's count in inact_count
+// (scanned into per-workgroup offsets before the scatter).
 template <int DIM, class PT>
 __global__ __launch_bounds__(256) void k_bin_count(KArgs A, unsigned *count, int *pkey, int *prank,
-                                                   long long inactive_key) {
+                                                   long long inactive_key, unsigned *inact_count) {
     constexpr int PB = Cfg<DIM>::PB;
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) *const_cast<int *>(A.npblk) = 0;
@@ -67,14 +72,26 @@ __global__ __launch_bounds__(256) void k_bin_count(KArgs A, unsigned *count, int
             A.act[p] = 0;  // leaves the domain: deactivated (default_points.cpp:217-219)
         }
     }
-    // wavefront aggregation: one atomic per distinct key in the wavefront
-    const int lane = threadIdx.x & 63;
-    unsigned long long pending = __ballot(live);
-    int rank = 0;
+    // inactive slots: rank inside the workgroup by ballot + LDS prefix
+    __shared__ unsigned wave_inact[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const bool inact = live && key == inactive_key;
+    const unsigned long long im = __ballot(inact);
+    if (lane == 0) wave_inact[wv] = (unsigned)__popcll(im);
+    __syncthreads();
+    unsigned before = 0, total = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+        if (w < wv) before += wave_inact[w];
+        total += wave_inact[w];
+    }
+    if (threadIdx.x == 0) inact_count[blockIdx.x] = total;
+    // active keys: wavefront aggregation, one atomic per distinct key in the wavefront
+    unsigned long long pending = __ballot(live && !inact);
+    int rank = inact ? (int)(before + __popcll(im & ((1ull << lane) - 1ull))) : 0;
     while (pending) {
         const int leader = __ffsll((long long)pending) - 1;
         const long long lkey = __shfl(key, leader);
-        const unsigned long long same = __ballot(live && key == lkey) & pending;
+        const unsigned long long same = __ballot(live && !inact && key == lkey) & pending;
         unsigned base = 0;
         if (lane == leader) base = atomicAdd(&count[lkey], (unsigned)__popcll(same));
         base = __shfl(base, leader);
@@ -154,25 +171,41 @@ __global__ __launch_bounds__(256) void k_clear(unsigned *count, long long n) {
 template <class E, int NF>
 __global__ __launch_bounds__(256) void k_bin_scatter(long long np, const E *P, E *P2, const int8_t *act,
                                                      int8_t *act2, const int *pid, int *pid2, const int *pkey,
-                                                     const int *prank, const unsigned *off) {
+                                                     const int *prank, const unsigned *off, int inactive_key,
+                                                     const unsigned *inact_off) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
-    const long long q = (long long)off[pkey[p]] + prank[p];
+    const int key = pkey[p];
+    long long q = (long long)off[key] + prank[p];
+    if (key == inactive_key) q += inact_off[blockIdx.x];  // same 256-slot blocks as k_bin_count
+    const int o = pid[p];
+    if (o < 0) {  // unused capacity: nothing to move
+        act2[q] = 0;
+        pid2[q] = -1;
+        return;
+    }
 #pragma unroll
     for (int f = 0; f < NF; f++) P2[f * np + q] = P[f * np + p];
     act2[q] = act[p];
-    pid2[q] = pid[p];
+    pid2[q] = o;
 }
 
-// slot order -> original order for selected planes (download)
+// slot order -> original order for selected planes (download, upload); slots of unused
+// capacity (pid < 0) are skipped. With pid_dst, also writes the identity pid there.
 template <class E>
 __global__ __launch_bounds__(256) void k_unsort(long long np, int nf, const E *src, long long src_stride, E *dst,
-                                                const int *pid) {
+                                                const int *pid, int *pid_dst = nullptr) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     const int o = pid[p];
     if (o < 0) return;
     for (int f = 0; f < nf; f++) dst[f * np + o] = src[f * src_stride + p];
+    if (pid_dst) pid_dst[o] = o;
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int *a, long long lo, long long hi, int v) {
+    const long long p = lo + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < hi) a[p] = v;
 }
 
 __global__ __launch_bounds__(256) void k_unsort_act8(long long np, const int8_t *act, int8_t *dst, const int *pid) {
@@ -182,18 +215,22 @@ __global__ __launch_bounds__(256) void k_unsort_act8(long long np, const int8_t 
     if (o >= 0) dst[o] = act[p];
 }
 
-__global__ __launch_bounds__(256) void k_iota(int *pid, long long np) {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < np) pid[p] = (int)p;
+__global__ __launch_bounds__(256) void k_iota_range(int *pid, long long lo, long long hi) {
+    const long long p = lo + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < hi) pid[p] = (int)p;
 }
 
-// deactivate the points whose original index lies in [lo, hi)
-__global__ __launch_bounds__(256) void k_deactivate(long long np, int8_t *act, const int *pid, long long lo,
+// release the points whose original index lies in [lo, hi): inactive and marked as
+// unused capacity (pid -1), so sorts and unsorts no longer move their planes
+__global__ __launch_bounds__(256) void k_deactivate(long long np, int8_t *act, int *pid, long long lo,
                                                     long long hi) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     const int o = pid[p];
-    if (o >= lo && o < hi) act[p] = 0;
+    if (o >= lo && o < hi) {
+        act[p] = 0;
+        pid[p] = -1;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_unsort_act(long long np, const int8_t *act, int *dst, const int *pid) {

@@ -97,11 +97,14 @@ def run_stream(pool: SlotPool, requests: Sequence[Tuple[W.Workload, int]], arriv
     finish = [None] * n
     results = [None] * n
     done = 0
+    t_step = t_admin = 0.0
+    chunks = 0
     while done < n:
         now = time.perf_counter() - t0
         while nxt < n and arrivals[nxt] <= now:
             queue.append(nxt)
             nxt += 1
+        ta = time.perf_counter()
         for s in pool.free_slots():
             if not queue:
                 break
@@ -109,13 +112,18 @@ def run_stream(pool: SlotPool, requests: Sequence[Tuple[W.Workload, int]], arriv
             w, steps = requests[r]
             pool.load(s, r, w, steps)
             start[r] = time.perf_counter() - t0
+        t_admin += time.perf_counter() - ta
         active = [s for s in range(pool.n_slots) if pool.req[s] is not None]
         if not active:
             if nxt < n:
                 time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
             continue
         k = min(chunk_max, min(pool.left[s] for s in active))
+        ts = time.perf_counter()
         pool.sim.step_batch(pool.dt, k)
+        t_step += time.perf_counter() - ts
+        chunks += 1
+        ta = time.perf_counter()
         for s in active:
             pool.left[s] -= k
             if pool.left[s] == 0:
@@ -128,8 +136,9 @@ def run_stream(pool: SlotPool, requests: Sequence[Tuple[W.Workload, int]], arriv
                 if log:
                     log(f"request {r} done: {requests[r][0].n_points} points x {requests[r][1]} steps, "
                         f"latency {finish[r] - arrivals[r]:.3f} s")
+        t_admin += time.perf_counter() - ta
     return {"arrival": list(arrivals), "start": start, "finish": finish, "results": results,
-            "makespan": time.perf_counter() - t0}
+            "makespan": time.perf_counter() - t0, "step_s": t_step, "admin_s": t_admin, "chunks": chunks}
 
 
 def poisson_arrivals(n, rate, seed=0xC0FFEE + 5):
