@@ -203,6 +203,34 @@ __device__ __forceinline__ void fetch_node(const KArgs &A, const SimDev &s, long
     }
 }
 
+// Occupancy hints (minimum waves per SIMD the register allocation must allow) and
+// software prefetch of the next point's state, per pass and form. Compile-time tuning
+// knobs; the defaults are the measured best.
+#ifndef MPM_WPE_G2P64
+#define MPM_WPE_G2P64 2
+#endif
+#ifndef MPM_WPE_P2G64
+#define MPM_WPE_P2G64 2
+#endif
+#ifndef MPM_WPE_G2P32
+#define MPM_WPE_G2P32 4
+#endif
+#ifndef MPM_WPE_P2G32
+#define MPM_WPE_P2G32 2
+#endif
+#ifndef MPM_PF_G2P
+#define MPM_PF_G2P 1
+#endif
+#ifndef MPM_PF_P2G
+#define MPM_PF_P2G 0
+#endif
+template <bool DO_G2P, bool DO_P2G, bool FP32> struct PassTune {
+    static constexpr int waves = (DO_G2P && DO_P2G) ? 2  // the fused pass
+                                 : DO_G2P ? (FP32 ? MPM_WPE_G2P32 : MPM_WPE_G2P64)
+                                          : (FP32 ? MPM_WPE_P2G32 : MPM_WPE_P2G64);
+    static constexpr bool prefetch = DO_G2P ? MPM_PF_G2P != 0 : MPM_PF_P2G != 0;
+};
+
 // Particle pass over one particle block per workgroup (persistent over the block list).
 //   DO_G2P: gather the grid of the current step (LDS patch), update the point
 //   DO_P2G: map the (updated) point and scatter it into the next step's accumulators
@@ -211,7 +239,9 @@ __device__ __forceinline__ void fetch_node(const KArgs &A, const SimDev &s, long
 // (Cfg::PSTR elements per plane); the accumulator patch is FP64 in both forms, the
 // gathered grid patch has the grid output's type.
 template <int DIM, bool DO_G2P, bool DO_P2G, bool NRM, class PT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_particles(KArgs A, int want_L) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PassTune<DO_G2P, DO_P2G, PT::FP32>::waves)))
+void k_particles(KArgs A, int want_L) {
+    constexpr bool PREFETCH = PassTune<DO_G2P, DO_P2G, PT::FP32>::prefetch;
     using R = typename PT::R;
     using GO = typename PT::GO;
     using St = typename PT::template State<DIM>;
@@ -219,7 +249,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     constexpr int NACC = NRM ? GF<DIM>::NACC : 1 + 2 * DIM;
     constexpr int NOUT = GF<DIM>::NOUT;
     constexpr int NZ = DIM == 3 ? 3 : 1;
-    __shared__ double lacc[DO_P2G ? NACC * PSTR : 1];
+    // P2G accumulator patch, FP64 in both forms. (Keeping the precision form's force partial
+    // sums in FP32 -- FF32 -- is supported but measured 5x slower on gfx950: FP32 LDS
+    // atomics (ds_add_f32) are much slower than ds_add_f64 there.)
+    constexpr bool FF32 = false;
+    constexpr int NACC64 = FF32 ? NACC - DIM : NACC;
+    __shared__ double lacc[DO_P2G ? NACC64 * PSTR : 1];
+    __shared__ float laccf[DO_P2G && FF32 ? DIM * PSTR : 1];
     __shared__ GO gpat[DO_G2P ? NOUT * PSTR : 1];
     __shared__ int bb[2 * DIM];
     const int tid = threadIdx.x;
@@ -239,7 +275,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         int org[DIM];
         patch_origin<DIM>(s, uniform(ent.key) - B.pblk_off, org);
         if (DO_P2G) {
-            for (int i = tid; i < NACC * PSTR; i += blockDim.x) lacc[i] = 0.0;
+            for (int i = tid; i < NACC64 * PSTR; i += blockDim.x) lacc[i] = 0.0;
+            if (FF32)
+                for (int i = tid; i < DIM * PSTR; i += blockDim.x) laccf[i] = 0.0f;
             if (tid < DIM) {
                 bb[tid] = 0x7fffffff;
                 bb[DIM + tid] = -0x7fffffff;
@@ -264,12 +302,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             }
         }
         __syncthreads();
+        // with PREFETCH the next point's state is loaded before this one is processed, so
+        // its loads are in flight during the LDS work (different slots: no ordering hazard)
+        St qn;
+        bool an = false;
+        if (PREFETCH && tid < count) {
+            an = A.act[start + tid] != 0;
+            if (an) PT::template load<DIM>(A, start + tid, qn, DO_P2G);
+        }
 #pragma unroll 1
         for (int k = tid; k < count; k += blockDim.x) {
             const long long p = start + k;
-            if (!A.act[p]) continue;
             St q;
-            PT::template load<DIM>(A, p, q, DO_P2G);
+            if (PREFETCH) {
+                const bool a = an;
+                q = qn;
+                const int kn = k + blockDim.x;
+                an = false;
+                if (kn < count) {
+                    an = A.act[start + kn] != 0;
+                    if (an) PT::template load<DIM>(A, start + kn, qn, DO_P2G);
+                }
+                if (!a) continue;
+            } else {
+                if (!A.act[p]) continue;
+                PT::template load<DIM>(A, p, q, DO_P2G);
+            }
             if (DO_G2P) {
                 Win<DIM, R> w;
                 bool cpdi;
@@ -413,13 +471,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
                                 for (int d = 0; d < DIM; d++) {
                                     atomicAdd(&lacc[(GF<DIM>::P + d) * PSTR + i], (double)(ap[d] * sx));
-                                    atomicAdd(&lacc[(GF<DIM>::F + d) * PSTR + i], (double)fma(af0[d], dx, af1[d] * sx));
+                                    if (FF32) atomicAdd(&laccf[d * PSTR + i], (float)fma(af0[d], dx, af1[d] * sx));
+                                    else atomicAdd(&lacc[(GF<DIM>::F + d) * PSTR + i], (double)fma(af0[d], dx, af1[d] * sx));
                                 }
                                 if (NRM && b1) {
-                                    atomicAdd(&lacc[(GF<DIM>::NRM + 0) * PSTR + i], (double)(an0 * dx));
-                                    atomicAdd(&lacc[(GF<DIM>::NRM + 1) * PSTR + i], (double)(an1 * sx));
-                                    if (DIM == 3)
-                                        atomicAdd(&lacc[(GF<DIM>::NRM + DIM - 1) * PSTR + i], (double)(an2 * sx));
+                                    constexpr int N0 = GF<DIM>::NRM - (FF32 ? DIM : 0);
+                                    atomicAdd(&lacc[(N0 + 0) * PSTR + i], (double)(an0 * dx));
+                                    atomicAdd(&lacc[(N0 + 1) * PSTR + i], (double)(an1 * sx));
+                                    if (DIM == 3) atomicAdd(&lacc[(N0 + DIM - 1) * PSTR + i], (double)(an2 * sx));
                                 }
                             }
                         }
@@ -451,7 +510,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                             const int i = patch_index<DIM>(nijk, org);
                             for (int f = 0; f < NACC; f++) {
                                 if (f >= 1 + 2 * DIM && !b1) continue;
-                                atomicAdd(&lacc[f * PSTR + i], (double)vals[f]);
+                                const bool isf = f >= GF<DIM>::F && f < GF<DIM>::F + DIM;
+                                if (FF32 && isf) atomicAdd(&laccf[(f - GF<DIM>::F) * PSTR + i], (float)vals[f]);
+                                else atomicAdd(&lacc[(FF32 && f >= GF<DIM>::NRM ? f - DIM : f) * PSTR + i], (double)vals[f]);
                             }
                         } else {
                             mark_range<DIM>(A, s, nijk, nijk);
@@ -485,7 +546,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
                 for (int f = 0; f < NACC; f++) {
                     if (f >= 1 + 2 * DIM && !b1) continue;
-                    const double v = lacc[f * PSTR + li];
+                    const bool isf = f >= GF<DIM>::F && f < GF<DIM>::F + DIM;
+                    const double v = (FF32 && isf) ? (double)laccf[(f - GF<DIM>::F) * PSTR + li]
+                                                   : lacc[(FF32 && f >= GF<DIM>::NRM ? f - DIM : f) * PSTR + li];
                     if (v != 0.0) unsafeAtomicAdd(&A.acc[f * A.nt + gi], v);
                 }
             }
