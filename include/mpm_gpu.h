@@ -59,12 +59,18 @@ typedef struct {
 } mpm_grid_desc;
 
 /* One body. Replaces: DefaultBody with DefaultPoints/DefaultNodes, the optional
- * IsotropicLinearElasticity {E, nu} (isolin.cpp:26-46) and CartesianBox (cartesian_box.cpp:23-60). */
+ * IsotropicLinearElasticity {E, nu} (isolin.cpp:26-46) and the body's boundary:
+ *   has_boundary 0  none
+ *                1  CartesianBox, all components fixed on the box faces (cartesian_box.cpp:23-60)
+ *                2  CartesianSmoothBox, the face-normal component fixed (cartesian_smooth_box.cpp:24-80)
+ *                3  CartesianFrictionalBox, face-normal fixed plus Coulomb friction with
+ *                   mu_f = bc_mu against the normal impulse (cartesian_frictional_box.cpp:23-151) */
 typedef struct {
     int64_t n_points;
     int has_material;
     double E, nu;
     int has_boundary;
+    double bc_mu;
 } mpm_body_desc;
 
 /* ContactHuang between two bodies of the same sim, {mu_f} and {body_1, body_2}

@@ -57,8 +57,11 @@ struct BodyDev {
     long long grid_base; // first node of the body's grid fields
     long long pblk_off;  // first particle-block key of the body
     long long npblk;     // particle blocks of the body
-    int has_material, has_boundary, is_b1, pad;
+    int has_material;
+    int has_boundary;    // 0 none, 1 box (all components), 2 smooth box (normal), 3 frictional box
+    int is_b1, pad;
     double twoG, lambda;
+    double bc_mu;        // Coulomb coefficient of the frictional box
 };
 
 struct ContactDev {

@@ -124,6 +124,8 @@ int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
         for (int b = 0; b < sd[i].nbody; b++) {
             const mpm_body_desc &bd = sd[i].bodies[b];
             if (bd.n_points < 0) return fail(MPM_ERR_ARG, "negative point count");
+            if (bd.has_boundary < 0 || bd.has_boundary > 3)
+                return fail(MPM_ERR_ARG, "sim %d body %d: has_boundary must be 0..3, got %d", i, b, bd.has_boundary);
             BodyDev B{};
             B.sim = i;
             B.local = b;
@@ -134,6 +136,7 @@ int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
             B.npblk = pnb;
             B.has_material = bd.has_material;
             B.has_boundary = bd.has_boundary;
+            B.bc_mu = bd.bc_mu;
             B.is_b1 = is_b1[b];
             if (bd.has_material) {
                 // IsotropicLinearElasticity::init (isolin.cpp:34-38)
@@ -881,9 +884,12 @@ int mpm_gpu_load_sim(mpm_gpu_handle *h, int sim, const mpm_sim_desc *d) {
     for (int k = 0; k < g.dim; k++)
         if (!(g.L[k] > 0) || g.N[k] < 1 || g.N[k] > h->grid_cap[sim][k])
             return fail(MPM_ERR_ARG, "sim %d: need L > 0 and 1 <= N <= %d in dimension %d", sim, h->grid_cap[sim][k], k);
-    for (int b = 0; b < d->nbody; b++)
+    for (int b = 0; b < d->nbody; b++) {
         if (d->bodies[b].n_points < 0 || d->bodies[b].n_points > h->lo.bodies[s.body0 + b].n)
             return fail(MPM_ERR_ARG, "sim %d body %d: at most %lld points", sim, b, (long long)h->lo.bodies[s.body0 + b].n);
+        if (d->bodies[b].has_boundary < 0 || d->bodies[b].has_boundary > 3)
+            return fail(MPM_ERR_ARG, "sim %d body %d: has_boundary must be 0..3, got %d", sim, b, d->bodies[b].has_boundary);
+    }
     for (int c = 0; c < d->ncontact; c++) {
         const mpm_contact_desc &cd = d->contacts[c];
         if (cd.body1 < 0 || cd.body1 >= d->nbody || cd.body2 < 0 || cd.body2 >= d->nbody || cd.body1 == cd.body2)
@@ -917,6 +923,7 @@ int mpm_gpu_load_sim(mpm_gpu_handle *h, int sim, const mpm_sim_desc *d) {
             const mpm_body_desc &bd = d->bodies[b];
             B.has_material = bd.has_material;
             B.has_boundary = bd.has_boundary;
+            B.bc_mu = bd.bc_mu;
             const double G = bd.E / (2.0 * (1.0 + bd.nu));
             const double K = bd.E / (3.0 * (1.0 - 2 * bd.nu));
             B.twoG = bd.has_material ? 2 * G : 0.0;

@@ -3,7 +3,7 @@
 //
 // One step (explicit_usl.cpp:48-87) is two or three launches over resident HBM state:
 //   k_grid       per touched node: body force m g, ContactHuang (contact_huang.cpp:104-197),
-//                CartesianBox (cartesian_box.cpp:73-85), moveGrid (explicit_usl.cpp:233-264);
+//                CartesianBox / SmoothBox / FrictionalBox (cartesian_*box.cpp), moveGrid (explicit_usl.cpp:233-264);
 //                one wavefront per 64-node block; consumes and re-zeroes the accumulators.
 //   k_particles  one workgroup per particle block (persistent over the block list):
 //                G2P from an LDS copy of the block's node patch -- movePoints,
@@ -618,12 +618,14 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
         const long long addr = lb * 64 + lane;
         if (valid) {
             const int nb = s.nbody;
-            double m[MAXB], p[MAXB][DIM], f[MAXB][DIM], nr[MAXB][DIM];
+            // f0: the force before contacts act, which the frictional box takes its normal
+            // impulse from (cartesian_frictional_box.cpp:102-117 recomputes it from S and gradS)
+            double m[MAXB], p[MAXB][DIM], f[MAXB][DIM], f0[MAXB][DIM], nr[MAXB][DIM];
 #pragma unroll
             for (int b = 0; b < MAXB; b++) {
                 m[b] = 0.0;
 #pragma unroll
-                for (int d = 0; d < DIM; d++) p[b][d] = f[b][d] = nr[b][d] = 0.0;
+                for (int d = 0; d < DIM; d++) p[b][d] = f[b][d] = f0[b][d] = nr[b][d] = 0.0;
                 if (b < nb) {
                     const BodyDev &B = A.bodies[s.body0 + b];
                     const long long gi = B.grid_base + addr;
@@ -634,6 +636,7 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
                         p[b][d] = A.acc[(G::P + d) * A.nt + gi];
                         // nodal body force m_i g (default_driver.cpp:109-119 via explicit_usl.cpp:149-154)
                         f[b][d] = fma(m[b], s.g[d], A.acc[(G::F + d) * A.nt + gi]);
+                        f0[b][d] = f[b][d];
                         A.acc[(G::P + d) * A.nt + gi] = 0.0;
                         A.acc[(G::F + d) * A.nt + gi] = 0.0;
                     }
@@ -718,14 +721,51 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
                     }
                 }
             }
-            // CartesianBox then moveGrid, per body
+            // the body's boundary, then moveGrid
 #pragma unroll
             for (int b = 0; b < MAXB; b++) {
                 if (b >= nb) continue;
                 const BodyDev &B = A.bodies[s.body0 + b];
-                if (B.has_boundary && wall) {
+                if (B.has_boundary == 1 && wall) {
+                    // CartesianBox::applyRules (cartesian_box.cpp:73-85): every component on a face node
 #pragma unroll
                     for (int d = 0; d < DIM; d++) p[b][d] = f[b][d] = 0.0;
+                } else if (B.has_boundary >= 2 && wall) {
+                    // CartesianSmoothBox::applyRules (cartesian_smooth_box.cpp:70-84): the component
+                    // normal to each face the node lies on. CartesianFrictionalBox::applyRules
+                    // (cartesian_frictional_box.cpp:119-143) also takes the closing momentum that
+                    // component would carry as the normal impulse.
+                    double dm[DIM];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        dm[d] = 0.0;
+                        const bool lo = ijk[d] == 0, hi = ijk[d] == s.N[d];
+                        if (lo || hi) {
+                            const double t = p[b][d] + dt * f0[b][d];
+                            if (lo) dm[d] = -(t < 0.0 ? t : 0.0);
+                            else dm[d] = -(t > 0.0 ? t : 0.0);
+                            p[b][d] = f[b][d] = 0.0;
+                        }
+                    }
+                    if (B.has_boundary == 3) {
+                        // Coulomb friction on the remaining motion, at most what stops it (145-150)
+                        double n2 = 0.0, l2 = 0.0, mv[DIM];
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) {
+                            n2 += dm[d] * dm[d];
+                            mv[d] = p[b][d] + dt * f[b][d];
+                            l2 += mv[d] * mv[d];
+                        }
+                        const double fnorm = sqrt(n2) / dt;
+                        const double len = sqrt(l2);
+                        if (len > 0.0) {
+                            const double a = len / dt, c = fnorm * B.bc_mu;
+                            const double mag = a < c ? a : c;
+                            const double r = 1.0 / len;
+#pragma unroll
+                            for (int d = 0; d < DIM; d++) f[b][d] -= (mag * mv[d]) * r;
+                        }
+                    }
                 }
                 const long long gi = B.grid_base + addr;
                 const bool has = m[b] > 0.0;

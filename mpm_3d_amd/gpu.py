@@ -49,7 +49,7 @@ class GridDesc(C.Structure):
 
 class BodyDesc(C.Structure):
     _fields_ = [("n_points", C.c_int64), ("has_material", C.c_int), ("E", C.c_double), ("nu", C.c_double),
-                ("has_boundary", C.c_int)]
+                ("has_boundary", C.c_int), ("bc_mu", C.c_double)]
 
 
 class ContactDesc(C.Structure):
@@ -139,6 +139,7 @@ def _sim_desc(w, keep):
         bds[bi].E = float(b.E or 0.0)
         bds[bi].nu = float(b.nu or 0.0)
         bds[bi].has_boundary = int(b.boundary)
+        bds[bi].bc_mu = float(getattr(b, "bc_mu", 0.0))
     cds = (ContactDesc * max(1, len(w.contacts)))()
     for ci, c in enumerate(w.contacts):
         cds[ci].body1, cds[ci].body2, cds[ci].mu = c.body1, c.body2, c.mu

@@ -74,9 +74,16 @@ void HipExplicitUSL::upload(Job *job) {
             b.nu = mat->nu;
         }
         if (bp->activeBoundary) {
-            if (!dynamic_cast<CartesianBox *>(bp->boundary.get()))
-                refuse(__func__, "HipExplicitUSL supports the CartesianBox boundary only.");
-            b.has_boundary = 1;
+            Boundary *bc = bp->boundary.get();
+            if (dynamic_cast<CartesianBox *>(bc)) b.has_boundary = 1;
+            else if (dynamic_cast<CartesianSmoothBox *>(bc)) b.has_boundary = 2;
+            else if (auto *fb = dynamic_cast<CartesianFrictionalBox *>(bc)) {
+                b.has_boundary = 3;
+                b.bc_mu = fb->mu_f;
+            } else {
+                refuse(__func__, "HipExplicitUSL supports the CartesianBox, CartesianSmoothBox and "
+                                 "CartesianFrictionalBox boundaries only.");
+            }
         }
         bd.push_back(b);
     }

@@ -69,7 +69,12 @@ struct IsotropicLinearElasticity : Material {
 struct Boundary : MPMObject {
     virtual void init(Job *, Body *) {}
 };
-struct CartesianBox : Boundary {};
+struct CartesianBox : Boundary {};        // every component fixed on the box faces
+struct CartesianSmoothBox : Boundary {};  // the face-normal component fixed (free slip)
+struct CartesianFrictionalBox : Boundary { // face-normal fixed, Coulomb friction mu_f
+    double mu_f = 0;
+    void init(Job *job, Body *body) override;
+};
 
 struct Body : MPMObject {
     int id = 0;

@@ -84,7 +84,11 @@ template <> Registry<Contact>::Registry() { table.insert(reg<Contact, ContactHua
 template <> Registry<Material>::Registry() {
     table.insert(reg<Material, IsotropicLinearElasticity>("IsotropicLinearElasticity"));
 }
-template <> Registry<Boundary>::Registry() { table.insert(reg<Boundary, CartesianBox>("CartesianBox")); }
+template <> Registry<Boundary>::Registry() {
+    table.insert(reg<Boundary, CartesianBox>("CartesianBox"));
+    table.insert(reg<Boundary, CartesianSmoothBox>("CartesianSmoothBox"));
+    table.insert(reg<Boundary, CartesianFrictionalBox>("CartesianFrictionalBox"));
+}
 
 // ------------------------------------------------------------------ configurator (config.cpp)
 Configurator::Configurator(std::string fileIN) : file(std::move(fileIN)) {
@@ -396,6 +400,17 @@ void CartesianLinear::init(Job *job) {
         L[d] = fp64_props[d];
         N[d] = int_props[d];
     }
+}
+
+// CartesianFrictionalBox::init (cartesian_frictional_box.cpp:26-37): mu_f is fp64-properties[0]
+void CartesianFrictionalBox::init(Job *, Body *) {
+    if (fp64_props.empty()) {
+        std::cout << fp64_props.size() << "\n";
+        fprintf(stderr, "%s:%s: Need at least 1 property defined (mu_f).\n", __FILE__, __func__);
+        exit(0);
+    }
+    mu_f = fp64_props[0];
+    std::cout << "Boundary properties ( mu_f = " << mu_f << ")." << std::endl;
 }
 
 void ContactHuang::init(Job *job) {

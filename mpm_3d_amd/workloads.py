@@ -34,7 +34,8 @@ class Body:
     active: np.ndarray     # (n,) int32
     E: Optional[float] = 1e5
     nu: Optional[float] = 0.3
-    boundary: bool = True
+    boundary: int = 1       # 0 none, 1 CartesianBox, 2 CartesianSmoothBox, 3 CartesianFrictionalBox
+    bc_mu: float = 0.0      # CartesianFrictionalBox mu_f
 
     @property
     def n(self) -> int:
@@ -128,6 +129,39 @@ def two_block_3d(n_cells=256, gap_cells=None, steps=1000, E=1e6, mu=0.4, gravity
         bodies.append(Body(name, m, v, x, xt, np.ones(len(m), np.int32), E, 0.3))
     return Workload(f"two_block_3d_{n_cells}^3", 3, L, N, bodies, [Contact(0, 1, mu)], 2e-5, steps,
                     gravity)
+
+
+def sliding_block(dim=2, boundary=3, mu=0.3, wall_dim=None, upper=False, speed=1.0, steps=500,
+                  E=1e6, rho=1000.0, g=9.81):
+    """An elastic block resting on one face of the domain box and sliding along it.
+
+    The face is ``wall_dim`` (default the last dimension) at 0, or at L when ``upper``;
+    gravity ``g`` points into it. The block moves at ``speed`` along the tangent
+    dimension(s): the other dimension in 2D, the diagonal of the two tangent dimensions in
+    3D. 2D: 40x40 grid on 1x1 m, a 0.2x0.1 m block (128 points); 3D: 32^3 grid on a 1 m
+    cube, a 0.25x0.25x0.125 m block (2048 points); 2 points per cell per dimension,
+    dt = 1e-4.
+    ``boundary`` and ``mu`` are the body's boundary type and friction coefficient."""
+    if wall_dim is None:
+        wall_dim = dim - 1
+    L = (1.0,) * dim
+    N = (40, 40) if dim == 2 else (32, 32, 32)
+    h = np.array([L[d] / N[d] for d in range(dim)])
+    tang = [d for d in range(dim) if d != wall_dim]
+    width, depth = (0.2, 0.1) if dim == 2 else (0.25, 0.125)
+    lo, hi = [0.0] * dim, [0.0] * dim
+    for d in tang:
+        lo[d], hi[d] = 0.3, 0.3 + width
+    lo[wall_dim], hi[wall_dim] = (L[wall_dim] - depth, L[wall_dim]) if upper else (0.0, depth)
+    vel = np.zeros(dim)
+    vel[tang] = speed / np.sqrt(len(tang))
+    m, v, x, xt = lattice_block(dim, h, lo, hi, 2, rho, velocity=vel)
+    body = Body("slider", m, v, x, xt, np.ones(len(m), np.int32), E, 0.3, boundary, mu)
+    grav = [0.0] * dim
+    grav[wall_dim] = g if upper else -g
+    face = f"{'upper' if upper else 'lower'}{wall_dim}"
+    return Workload(f"sliding_block_{dim}d_{face}_bc{boundary}_mu{mu}", dim, L, N, [body], [], 1e-4, steps,
+                    tuple(grav))
 
 
 def column_3d(wide=True, steps=1000, seed=0xC0FFEE + 2, grid=(256, 256, 128), domain=(2.0, 2.0, 1.0),

@@ -23,7 +23,10 @@
  * finite differences, mass/momentum conservation of P2G, free fall and rigid translation
  * trajectories in closed form, a linear velocity field (L, density, Hooke stress),
  * hand-computed Jaumann updates (uniaxial, volumetric, shear, spin), the CartesianBox mask,
- * and ContactHuang's momentum conservation, normals and friction limits.
+ * ContactHuang's momentum conservation, normals and friction limits, and the smooth and
+ * frictional boxes (face-normal mask against a boundary-free twin, free-slip momentum
+ * conservation, mu = 0 equal to the smooth box, the Coulomb impulse per face node, and a
+ * sliding block's deceleration near mu g).
  */
 #define _GNU_SOURCE
 #include "mpm_oracle.h"
@@ -80,8 +83,11 @@ typedef struct {
 
     int has_material;
     double E, nu, Gmod, K, lambda;
-    int has_boundary;
-    double *bc_mask; /* 3 per node (cartesian_box.cpp:53-59) */
+    int has_boundary;  /* 1 CartesianBox, 2 CartesianSmoothBox, 3 CartesianFrictionalBox */
+    double bc_mu;      /* CartesianFrictionalBox mu_f */
+    double *bc_mask;   /* 3 per node (cartesian_box.cpp:53-59, cartesian_smooth_box.cpp:40-50,
+                          cartesian_frictional_box.cpp:55-77) */
+    double *bc_force;  /* 3 per node: bcNodalForce of the frictional box */
 
     /* DefaultNodes, dense over the grid (default_nodes.cpp:29-63) */
     double *nm;                                  /* node_count */
@@ -562,6 +568,10 @@ static void map_points_to_nodes(ora_sim *s, ora_body *B) {
         for (int d = 0; d < D; d++) B->nf[3 * i + d] = B->nf[3 * i + d] - nvec[3 * i + d];
     free(nvec);
     free(tmpMat);
+    /* CartesianFrictionalBox::generateRules/applyRules recompute exactly this force with the
+     * same operations in the same order before contacts act (cartesian_frictional_box.cpp:
+     * 85-117); keep it */
+    if (B->has_boundary == 3) memcpy(B->bc_force, B->nf, sizeof(double) * 3 * nn);
 }
 
 static double dot3(const double *a, const double *b, int D) {
@@ -630,7 +640,8 @@ static void contact_apply(ora_sim *s, ora_contact *C, int spec) {
     }
 }
 
-/* CartesianBox::applyRules (cartesian_box.cpp:73-85) */
+/* CartesianBox::applyRules (cartesian_box.cpp:73-85) and CartesianSmoothBox::applyRules
+ * (cartesian_smooth_box.cpp:70-84): zero the masked components */
 static void boundary_apply(ora_sim *s, ora_body *B) {
     const int D = s->dim;
     for (int64_t i = 0; i < s->node_count; i++) {
@@ -640,6 +651,51 @@ static void boundary_apply(ora_sim *s, ora_body *B) {
                 B->nmxt[3 * i + d] = 0;
                 B->nf[3 * i + d] = 0;
             }
+        }
+    }
+}
+
+static double norm3(const double *a, int D) {
+    double t = 0;
+    for (int i = 0; i < D; i++) t += a[i] * a[i];
+    return sqrt(t);
+}
+
+/* CartesianFrictionalBox::applyRules (cartesian_frictional_box.cpp:98-151). bcNodalForce
+ * = S (m b) - gradS (V T) (105-117) is the P2G force before contact, kept in bc_force. */
+static void frictional_apply(ora_sim *s, ora_body *B) {
+    const int D = s->dim;
+    const double dt = s->dt;
+    for (int64_t i = 0; i < s->node_count; i++) {
+        double dm[3] = {0, 0, 0};
+        double *mxt = &B->nmxt[3 * i], *f = &B->nf[3 * i];
+        const double *F = &B->bc_force[3 * i];
+        for (int d = 0; d < D; d++) {
+            const double mk = B->bc_mask[3 * i + d];
+            if (mk == -1) {
+                const double t = mxt[d] + dt * F[d];
+                dm[d] = -(t < 0.0 ? t : 0.0);  /* -min(0, .) */
+                B->nxt[3 * i + d] = 0;
+                mxt[d] = 0;
+                f[d] = 0;
+            } else if (mk == 1) {
+                const double t = mxt[d] + dt * F[d];
+                dm[d] = -(t > 0.0 ? t : 0.0);  /* -max(0, .) */
+                B->nxt[3 * i + d] = 0;
+                mxt[d] = 0;
+                f[d] = 0;
+            } else {
+                dm[d] = 0;
+            }
+        }
+        const double f_normal = norm3(dm, D) / dt;
+        for (int d = 0; d < D; d++) dm[d] = mxt[d] + dt * f[d];
+        const double len = norm3(dm, D);
+        if (len > 0) {
+            const double a = len / dt, b = f_normal * B->bc_mu;
+            const double mag = a < b ? a : b;
+            const double r = 1.0 / len;
+            for (int d = 0; d < D; d++) f[d] -= (mag * dm[d]) * r;
         }
     }
 }
@@ -743,8 +799,11 @@ static void usl_step(ora_sim *s) {
     /* generateLoads/applyLoads: no-ops for DefaultBody/DefaultPoints/DefaultNodes */
     for (int c = 0; c < s->ncontact; c++) contact_generate(s, &s->contacts[c]);
     for (int c = 0; c < s->ncontact; c++) contact_apply(s, &s->contacts[c], s->contact_spec);
-    for (int b = 0; b < s->nbody; b++)
-        if (s->bodies[b].has_boundary) boundary_apply(s, &s->bodies[b]);
+    for (int b = 0; b < s->nbody; b++) {
+        ora_body *B = &s->bodies[b];
+        if (B->has_boundary == 3) frictional_apply(s, B);
+        else if (B->has_boundary) boundary_apply(s, B);
+    }
     for (int b = 0; b < s->nbody; b++) move_grid(s, &s->bodies[b]);
     for (int b = 0; b < s->nbody; b++) move_points(s, &s->bodies[b]);
     for (int b = 0; b < s->nbody; b++) strain_rate(s, &s->bodies[b]);
@@ -790,7 +849,7 @@ ora_sim *ora_create(int dim, const double L[3], const int N[3], int cpdi_spec, i
 
 int ora_add_body(ora_sim *s, int64_t n, const double *m, const double *v, const double *x,
                  const double *xt, const int32_t *active, int has_material, double E, double nu,
-                 int has_boundary) {
+                 int has_boundary, double bc_mu) {
     s->bodies = (ora_body *)realloc(s->bodies, sizeof(ora_body) * (s->nbody + 1));
     ora_body *B = &s->bodies[s->nbody];
     memset(B, 0, sizeof(*B));
@@ -816,6 +875,8 @@ int ora_add_body(ora_sim *s, int64_t n, const double *m, const double *v, const 
     B->E = E;
     B->nu = nu;
     B->has_boundary = has_boundary;
+    B->bc_mu = bc_mu;
+    if (has_boundary < 0 || has_boundary > 3) return -1;
     B->S.ncomp = 1;
     B->G.ncomp = 3;
     return s->nbody++;
@@ -861,7 +922,8 @@ int ora_init(ora_sim *s) {
             B->K = B->E / (3.0 * (1.0 - 2 * B->nu));
             B->lambda = B->K - 2.0 * B->Gmod / 3.0;
         }
-        /* CartesianBox::init (cartesian_box.cpp:23-60) */
+        /* CartesianBox::init (cartesian_box.cpp:23-60), CartesianSmoothBox::init
+         * (cartesian_smooth_box.cpp:22-54), CartesianFrictionalBox::init (20-80) */
         if (B->has_boundary) {
             double Lx[3] = {0, 0, 0};
             for (int64_t i = 0; i < nn; i++)
@@ -869,13 +931,32 @@ int ora_init(ora_sim *s) {
                     if (B->nx[3 * i + d] > Lx[d]) Lx[d] = B->nx[3 * i + d];
             B->bc_mask = (double *)calloc(3 * nn, sizeof(double));
             for (int64_t i = 0; i < nn; i++) {
-                for (int d = 0; d < s->dim; d++) {
-                    if (B->nx[3 * i + d] == 0 || B->nx[3 * i + d] == Lx[d]) {
-                        for (int e = 0; e < s->dim; e++) B->bc_mask[3 * i + e] = 1;
-                        break;
+                if (B->has_boundary == 1) {
+                    for (int d = 0; d < s->dim; d++) {
+                        if (B->nx[3 * i + d] == 0 || B->nx[3 * i + d] == Lx[d]) {
+                            for (int e = 0; e < s->dim; e++) B->bc_mask[3 * i + e] = 1;
+                            break;
+                        }
                     }
+                } else if (B->has_boundary == 2) {
+                    for (int d = 0; d < s->dim; d++)
+                        if (B->nx[3 * i + d] == 0 || B->nx[3 * i + d] == Lx[d]) B->bc_mask[3 * i + d] = 1;
+                } else {
+                    int edge = 0;
+                    for (int d = 0; d < s->dim; d++) {
+                        if (B->nx[3 * i + d] == 0) {
+                            B->bc_mask[3 * i + d] = -1;
+                            edge = 1;
+                        } else if (B->nx[3 * i + d] == Lx[d]) {
+                            B->bc_mask[3 * i + d] = 1;
+                            edge = 1;
+                        }
+                    }
+                    for (int d = 0; d < s->dim; d++)
+                        if (edge && B->bc_mask[3 * i + d] != 1 && B->bc_mask[3 * i + d] != -1) B->bc_mask[3 * i + d] = 2;
                 }
             }
+            if (B->has_boundary == 3) B->bc_force = (double *)calloc(3 * nn, sizeof(double));
         }
     }
     for (int c = 0; c < s->ncontact; c++) {
@@ -1015,7 +1096,7 @@ void ora_destroy(ora_sim *s) {
         ora_body *B = &s->bodies[b];
         free(B->x); free(B->u); free(B->xt); free(B->mxt); free(B->b);
         free(B->L); free(B->T); free(B->m); free(B->v); free(B->v0); free(B->extent);
-        free(B->active); free(B->bc_mask);
+        free(B->active); free(B->bc_mask); free(B->bc_force);
         free(B->nm); free(B->nx); free(B->nu_); free(B->nxt); free(B->ndxt); free(B->nmxt); free(B->nf);
         coo_free(&B->S);
         coo_free(&B->G);

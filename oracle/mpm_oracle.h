@@ -46,9 +46,11 @@ typedef struct ora_sim ora_sim;
 
 ora_sim *ora_create(int dim, const double L[3], const int N[3], int cpdi_spec, int contact_spec);
 /* returns the body index; arrays are copied. x, xt are n*3 AoS. E/nu used if has_material. */
+/* has_boundary: 0 none, 1 CartesianBox, 2 CartesianSmoothBox, 3 CartesianFrictionalBox
+ * (with mu_f = bc_mu) */
 int ora_add_body(ora_sim *s, int64_t n, const double *m, const double *v, const double *x,
                  const double *xt, const int32_t *active, int has_material, double E, double nu,
-                 int has_boundary);
+                 int has_boundary, double bc_mu);
 int ora_add_contact(ora_sim *s, int body1, int body2, double mu);
 /* Job::init order: points (extent, corner table), nodes, material, boundary, contacts. */
 int ora_init(ora_sim *s);

@@ -40,7 +40,7 @@ def _lib(native: bool = False):
     lib.ora_create.argtypes = [C.c_int, _dp, C.POINTER(C.c_int), C.c_int, C.c_int]
     lib.ora_add_body.restype = C.c_int
     lib.ora_add_body.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp, _dp, _ip, C.c_int,
-                                 C.c_double, C.c_double, C.c_int]
+                                 C.c_double, C.c_double, C.c_int, C.c_double]
     lib.ora_add_contact.restype = C.c_int
     lib.ora_add_contact.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
     lib.ora_init.restype = C.c_int
@@ -101,7 +101,9 @@ class OracleSim:
         self.sizes = []
         self._keep = []
 
-    def add_body(self, m, v, x, xt, active=None, E=None, nu=None, boundary=True):
+    def add_body(self, m, v, x, xt, active=None, E=None, nu=None, boundary=True, bc_mu=0.0):
+        """boundary: 0/False none, 1/True CartesianBox, 2 CartesianSmoothBox,
+        3 CartesianFrictionalBox with mu_f = bc_mu"""
         m = np.ascontiguousarray(m, dtype=np.float64)
         n = m.shape[0]
         v = np.ascontiguousarray(v, dtype=np.float64)
@@ -110,7 +112,7 @@ class OracleSim:
         act = np.ones(n, np.int32) if active is None else np.ascontiguousarray(active, dtype=np.int32)
         has_mat = E is not None
         b = self.lib.ora_add_body(self.h, n, _d(m), _d(v), _d(x3), _d(xt3), _i(act), int(has_mat),
-                                  float(E or 0.0), float(nu or 0.0), int(boundary))
+                                  float(E or 0.0), float(nu or 0.0), int(boundary), float(bc_mu))
         self.sizes.append(n)
         return b
 

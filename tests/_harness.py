@@ -30,7 +30,7 @@ def oracle_of(w, native=False):
     from oracle.pyoracle import OracleSim
     s = OracleSim(w.dim, w.L, w.N, w.cpdi_spec, w.contact_spec, native=native)
     for b in w.bodies:
-        s.add_body(b.m, b.v, b.x, b.xt, b.active, E=b.E, nu=b.nu, boundary=b.boundary)
+        s.add_body(b.m, b.v, b.x, b.xt, b.active, E=b.E, nu=b.nu, boundary=b.boundary, bc_mu=b.bc_mu)
     for c in w.contacts:
         s.add_contact(c.body1, c.body2, c.mu)
     s.init(g=w.gravity)

@@ -76,3 +76,19 @@ def test_null_handle_is_an_argument_error():
     assert lib.mpm_gpu_step(None, 1e-4, 1) == -1
     assert b"NULL" in lib.mpm_gpu_last_error() or b"null" in lib.mpm_gpu_last_error()
     assert lib.mpm_gpu_destroy(None) in (0, -1)
+
+
+def test_body_descriptor_layout_and_boundary_check():
+    """The ctypes BodyDesc has the C struct's stride (a wrong stride would read the second
+    body's fields at the wrong offset), and has_boundary outside 0..3 is rejected with a
+    message naming the body."""
+    lib = G.load_library()
+    assert C.sizeof(G.BodyDesc) == 48
+    w = W.two_block_3d(n_cells=16, gap_cells=1)
+    d, keep = _sim_desc(w)
+    keep[0][1].has_boundary = 3
+    keep[0][1].bc_mu = 0.3
+    assert lib.mpm_gpu_footprint(C.byref(d), 1, G.MPM_PREC_FP64) > 0
+    keep[0][1].has_boundary = 4
+    assert lib.mpm_gpu_footprint(C.byref(d), 1, G.MPM_PREC_FP64) < 0
+    assert b"body 1: has_boundary must be 0..3" in lib.mpm_gpu_last_error()

@@ -64,16 +64,17 @@ body
     material-class = "{mat}"
     material-props = {{{props}}}
     material-props = {{{props}}}
-    boundary-class = "CartesianBox"
+    boundary-class = "{bc}"{bc_props}
     colour = "blue"
 }}
 """
 
 
-def _write(tmp_path, mat="IsotropicLinearElasticity", props="1e5, 0.3"):
+def _write(tmp_path, mat="IsotropicLinearElasticity", props="1e5, 0.3", bc="CartesianBox", bc_props=None):
     (tmp_path / "p.points").write_text("2\n1 0.01 0.5 0.5 0 0 1\n1 0.01 0.25 0.5 0.1 0 1\n")
     cfg = tmp_path / "t.cfg"
-    cfg.write_text(BASE.format(mat=mat, props=props))
+    extra = "" if bc_props is None else f"\n    boundary-props = {{{bc_props}}}"
+    cfg.write_text(BASE.format(mat=mat, props=props, bc=bc, bc_props=extra))
     return str(cfg)
 
 
@@ -103,3 +104,21 @@ def test_bad_material_properties_exit_zero_with_message(tmp_path):
 def test_missing_config_file():
     r = run(["-c", "/nonexistent/x.cfg", "--check"])
     assert r.returncode == 0 and "Unable to open file" in r.stdout
+
+
+@pytest.mark.parametrize("bc,props", [("CartesianSmoothBox", None), ("CartesianFrictionalBox", "0.35")])
+def test_box_boundaries_configure(tmp_path, bc, props):
+    """cartesian_smooth_box.cpp / cartesian_frictional_box.cpp are registered; the
+    frictional box reads mu_f from boundary-props[0] (cartesian_frictional_box.cpp:26-37)."""
+    r = run(["-c", _write(tmp_path, bc=bc, bc_props=props), "--check"], cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    assert "Job Initialized" in r.stdout
+    if props:
+        assert "Boundary properties ( mu_f = 0.35)." in r.stdout
+
+
+def test_frictional_box_without_mu_exits_zero_with_message(tmp_path):
+    r = run(["-c", _write(tmp_path, bc="CartesianFrictionalBox"), "--check"], cwd=str(tmp_path))
+    assert r.returncode == 0
+    assert "Need at least 1 property defined (mu_f)." in r.stderr
+    assert "Job Initialized" not in r.stdout
