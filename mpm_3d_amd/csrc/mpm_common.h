@@ -86,6 +86,7 @@ template <> struct Cfg<3> {
     static constexpr int ZS = 68;  // z-plane stride of the LDS patch
     static constexpr int PATCH = 512;   // nodes in the patch
     static constexpr int PSTR = 544;    // doubles per field plane of the patch (8 * ZS)
+    static constexpr int PBLKS = 8;     // node blocks covering the patch (2 x 2 x 2)
     static constexpr int NTS = 6;
 };
 template <> struct Cfg<2> {
@@ -96,6 +97,7 @@ template <> struct Cfg<2> {
     static constexpr int ZS = 0;
     static constexpr int PATCH = 144;
     static constexpr int PSTR = 144;
+    static constexpr int PBLKS = 4;     // node blocks covering the patch (2 x 2, the second ones in part)
     static constexpr int NTS = 4;
 };
 
@@ -195,6 +197,33 @@ __host__ __device__ __forceinline__ long long node_addr(const SimDev &s, const i
         loc += (i2 & (NB - 1)) << (2 * L);
     }
     return blk * 64 + loc;
+}
+
+// Node i of the LDS patch counted by whole node blocks: the patch of origin org (a node-
+// block boundary in storage) is covered by Cfg::PBLKS blocks of 64 nodes; i = 64 * block
+// + lane. Sets the node's sim-local ijk, inp (the node lies in the patch) and in (it lies in
+// the domain, 0 <= ijk <= N); returns its storage offset (meaningful when in).
+template <int DIM>
+__device__ __forceinline__ long long patch_node(const SimDev &s, const int *org, int i, int *nijk, bool &inp,
+                                                bool &in) {
+    constexpr int NB = Cfg<DIM>::NB, L = Cfg<DIM>::LNB, P = Cfg<DIM>::P;
+    constexpr int BPD = (P + NB - 1) / NB;  // node blocks per patch edge
+    const int b = i >> 6, lane = i & 63;
+    int bc[3] = {b % BPD, (b / BPD) % BPD, b / (BPD * BPD)};
+    int pl[3] = {bc[0] * NB + (lane & (NB - 1)), bc[1] * NB + ((lane >> L) & (NB - 1)), bc[2] * NB + (lane >> (2 * L))};
+    inp = true;
+    in = true;
+    long long blk = 0, stride = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        nijk[d] = org[d] + pl[d];
+        inp = inp && pl[d] < P;
+        in = in && nijk[d] >= 0 && nijk[d] <= s.N[d];
+        blk += (long long)((org[d] + NSHIFT) / NB + bc[d]) * stride;
+        stride *= s.NBn[d];
+    }
+    in = in && inp;
+    return blk * 64 + lane;
 }
 
 // CartesianLinear::inDomain (cartesian.cpp:245-253): 0 <= x <= L in every dimension

@@ -144,16 +144,18 @@ template <class R> struct RowW {
     R Su, gx, gy, gz;
 };
 
-template <int DIM, class R>
-__device__ __forceinline__ RowW<R> row_weights(const Win<DIM, R> &w, int c1, int c2, R sscale, R gscale) {
-    const R sz = DIM == 3 ? w.sw[DIM - 1][c2] : (R)1;
-    const R dz = DIM == 3 ? w.dw[DIM - 1][c2] : (R)0;
-    const R syz = w.sw[1][c1] * sz;
-    RowW<R> r;
+// (D: the arithmetic type of the result; the window's weights are converted to it)
+template <int DIM, class R, class D = R>
+__device__ __forceinline__ RowW<D> row_weights(const Win<DIM, R> &w, int c1, int c2, D sscale, D gscale) {
+    const D sz = DIM == 3 ? (D)w.sw[DIM - 1][c2] : (D)1;
+    const D dz = DIM == 3 ? (D)w.dw[DIM - 1][c2] : (D)0;
+    const D sy = (D)w.sw[1][c1], dy = (D)w.dw[1][c1];
+    const D syz = sy * sz;
+    RowW<D> r;
     r.Su = syz * sscale;
     r.gx = syz * gscale;
-    r.gy = (w.dw[1][c1] * sz) * gscale;
-    r.gz = (w.sw[1][c1] * dz) * gscale;
+    r.gy = (dy * sz) * gscale;
+    r.gz = (sy * dz) * gscale;
     return r;
 }
 
@@ -249,6 +251,7 @@ void k_particles(KArgs A, int want_L) {
     constexpr int NACC = NRM ? GF<DIM>::NACC : 1 + 2 * DIM;
     constexpr int NOUT = GF<DIM>::NOUT;
     constexpr int NZ = DIM == 3 ? 3 : 1;
+    constexpr int PBLKS = Cfg<DIM>::PBLKS;
     // P2G accumulator patch, FP64 in both forms. (Keeping the precision form's force partial
     // sums in FP32 -- FF32 -- is supported but measured 5x slower on gfx950: FP32 LDS
     // atomics (ds_add_f32) are much slower than ds_add_f64 there.)
@@ -284,24 +287,28 @@ void k_particles(KArgs A, int want_L) {
             }
         }
         if (DO_G2P) {
-            // stage the block's node patch: reads along node blocks, one field plane each
-            for (int i = tid; i < NOUT * PATCH; i += blockDim.x) {
-                const int f = i / PATCH, li = i - f * PATCH;
+            // stage the block's node patch. The patch starts on a node-block boundary, so it
+            // is read as whole 64-node blocks: one node per thread, all its fields at once.
+            for (int i = tid; i < PBLKS * 64; i += blockDim.x) {
                 int nijk[DIM];
-                int r = li;
-                bool in = true;
+                bool inp, in;
+                const long long gi = gbase + patch_node<DIM>(s, org, i, nijk, inp, in);
+                if (inp) {
+                    const int li = patch_index<DIM>(nijk, org);
 #pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    nijk[d] = org[d] + r % P;
-                    r /= P;
-                    in = in && nijk[d] >= 0 && nijk[d] <= s.N[d];
+                    for (int f = 0; f < NOUT; f++) gpat[f * PSTR + li] = in ? PT::fetch(A, f * A.nt + gi) : (GO)0;
                 }
-                GO v = 0;
-                if (in) v = PT::fetch(A, f * A.nt + gbase + node_addr<DIM>(s, nijk));
-                gpat[f * PSTR + patch_index<DIM>(nijk, org)] = v;
             }
         }
         __syncthreads();
+        // hull of the LDS-path windows, per thread (min / max node index per dimension);
+        // combined over the workgroup once per particle block
+        int tlo[DIM], thi[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            tlo[d] = 0x7fffffff;
+            thi[d] = -0x7fffffff;
+        }
         // with PREFETCH the next point's state is loaded before this one is processed, so
         // its loads are in flight during the LDS work (different slots: no ordering hazard)
         St qn;
@@ -441,44 +448,61 @@ void k_particles(KArgs A, int want_L) {
                 if (mode == MODE_SEP && window_in_patch<DIM>(w, org)) {
 #pragma unroll
                     for (int d = 0; d < DIM; d++) {
-                        atomicMin(&bb[d], w.base[d]);
-                        atomicMax(&bb[DIM + d], w.base[d] + 2);
+                        tlo[d] = min(tlo[d], w.base[d]);
+                        thi[d] = max(thi[d], w.base[d] + 2);
                     }
-                    // row-factored scatter: along a row every field is a * sw0 + b * dw0
-                    const R gscale = (R)1 / ((R)(1 << DIM) * (R)0.5 * q.ext);
+                    // row-factored scatter: along a row every field is a * sw0 + b * dw0. The
+                    // node values are formed in FP64 from the point's values (in the precision
+                    // form too: scalar FP32 and FP64 FMAs issue at one rate, and the LDS
+                    // accumulators are FP64, so this saves a conversion per value)
+                    using D = double;
+                    const D gscale = (D)((R)1 / ((R)(1 << DIM) * (R)0.5 * q.ext));
+                    const D dm = pv.m;
+                    D dmv[DIM], dVT[DIM][DIM], sx0[3], dx0[3];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        dmv[d] = pv.mv[d];
+#pragma unroll
+                        for (int c = 0; c < DIM; c++) dVT[d][c] = pv.VT[d][c];
+                    }
+#pragma unroll
+                    for (int c0 = 0; c0 < 3; c0++) {
+                        sx0[c0] = w.sw[0][c0];
+                        dx0[c0] = w.dw[0][c0];
+                    }
                     const int i0 = patch_index<DIM>(w.base, org);
 #pragma unroll
                     for (int c2 = 0; c2 < NZ; c2++) {
 #pragma unroll
                         for (int c1 = 0; c1 < 3; c1++) {
-                            const RowW<R> rw = row_weights<DIM, R>(w, c1, c2, sscale, gscale);
-                            const R am = pv.m * rw.Su;
-                            R ap[DIM], af0[DIM], af1[DIM];
+                            const RowW<D> rw = row_weights<DIM, R, D>(w, c1, c2, (D)sscale, gscale);
+                            const D am = dm * rw.Su;
+                            D ap[DIM], af0[DIM], af1[DIM];
 #pragma unroll
                             for (int d = 0; d < DIM; d++) {
-                                ap[d] = pv.mv[d] * rw.Su;
-                                af0[d] = -(pv.VT[d][0] * rw.gx);
-                                R t = pv.VT[d][1] * rw.gy;
-                                if (DIM == 3) t = fma(pv.VT[d][DIM - 1], rw.gz, t);
+                                ap[d] = dmv[d] * rw.Su;
+                                af0[d] = -(dVT[d][0] * rw.gx);
+                                D t = dVT[d][1] * rw.gy;
+                                if (DIM == 3) t = fma(dVT[d][DIM - 1], rw.gz, t);
                                 af1[d] = -t;
                             }
-                            const R an0 = pv.m * rw.gx, an1 = pv.m * rw.gy, an2 = pv.m * rw.gz;
+                            const D an0 = dm * rw.gx, an1 = dm * rw.gy, an2 = dm * rw.gz;
 #pragma unroll
                             for (int c0 = 0; c0 < 3; c0++) {
                                 const int i = i0 + c0 + P * c1 + ZS * c2;
-                                const R sx = w.sw[0][c0], dx = w.dw[0][c0];
-                                atomicAdd(&lacc[GF<DIM>::M * PSTR + i], (double)(am * sx));
+                                const D sx = sx0[c0], dx = dx0[c0];
+                                atomicAdd(&lacc[GF<DIM>::M * PSTR + i], am * sx);
 #pragma unroll
                                 for (int d = 0; d < DIM; d++) {
-                                    atomicAdd(&lacc[(GF<DIM>::P + d) * PSTR + i], (double)(ap[d] * sx));
+                                    atomicAdd(&lacc[(GF<DIM>::P + d) * PSTR + i], ap[d] * sx);
                                     if (FF32) atomicAdd(&laccf[d * PSTR + i], (float)fma(af0[d], dx, af1[d] * sx));
-                                    else atomicAdd(&lacc[(GF<DIM>::F + d) * PSTR + i], (double)fma(af0[d], dx, af1[d] * sx));
+                                    else atomicAdd(&lacc[(GF<DIM>::F + d) * PSTR + i], fma(af0[d], dx, af1[d] * sx));
                                 }
                                 if (NRM && b1) {
                                     constexpr int N0 = GF<DIM>::NRM - (FF32 ? DIM : 0);
-                                    atomicAdd(&lacc[(N0 + 0) * PSTR + i], (double)(an0 * dx));
-                                    atomicAdd(&lacc[(N0 + 1) * PSTR + i], (double)(an1 * sx));
-                                    if (DIM == 3) atomicAdd(&lacc[(N0 + DIM - 1) * PSTR + i], (double)(an2 * sx));
+                                    atomicAdd(&lacc[(N0 + 0) * PSTR + i], an0 * dx);
+                                    atomicAdd(&lacc[(N0 + 1) * PSTR + i], an1 * sx);
+                                    if (DIM == 3) atomicAdd(&lacc[(N0 + DIM - 1) * PSTR + i], an2 * sx);
                                 }
                             }
                         }
@@ -502,8 +526,8 @@ void k_particles(KArgs A, int want_L) {
                         if (node_in_patch<DIM>(nijk, org)) {
 #pragma unroll
                             for (int d = 0; d < DIM; d++) {
-                                atomicMin(&bb[d], nijk[d]);
-                                atomicMax(&bb[DIM + d], nijk[d]);
+                                tlo[d] = min(tlo[d], nijk[d]);
+                                thi[d] = max(thi[d], nijk[d]);
                             }
                             R vals[GF<DIM>::NACC];
                             p2g_node_values<DIM, R>(pv, S, G, vals, b1);
@@ -523,6 +547,13 @@ void k_particles(KArgs A, int want_L) {
             }
         }
         if (DO_P2G) {
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                if (tlo[d] <= thi[d]) {
+                    atomicMin(&bb[d], tlo[d]);
+                    atomicMax(&bb[DIM + d], thi[d]);
+                }
+            }
             __syncthreads();
             // flush the touched part of the patch and mark its node blocks
             int lo[DIM], ex[DIM];
