@@ -133,7 +133,7 @@ def run_batched(args):
     steps each; against the same simulations run one at a time (timed on a sample)."""
     from mpm_3d_amd import workloads as W
     from mpm_3d_amd.gpu import MPM_OPT_AUTOTUNE, MPM_OPT_GRAPH, MPM_OPT_GRAPH_CAPTURES, MPM_OPT_GRID_WGS, \
-        MPM_OPT_PASS_GRID, MPM_PREC_FP32, GpuSim
+        MPM_OPT_P2G_GRID, MPM_OPT_PASS_GRID, MPM_PREC_FP32, GpuSim
     ws = W.ensemble_2d(n_sims=512, steps=args.ens_steps)
     npts = sum(w.n_points for w in ws)
     sim = GpuSim(ws, precision=MPM_PREC_FP32)
@@ -144,7 +144,8 @@ def run_batched(args):
     sim.step_batch([w.dt for w in ws], args.ens_steps)
     sim.synchronize()
     wall = time.perf_counter() - t0
-    tuned = {"pass_grid": sim.get_option(MPM_OPT_PASS_GRID), "grid_wgs": sim.get_option(MPM_OPT_GRID_WGS),
+    tuned = {"pass_grid": sim.get_option(MPM_OPT_PASS_GRID), "p2g_grid": sim.get_option(MPM_OPT_P2G_GRID),
+             "grid_wgs": sim.get_option(MPM_OPT_GRID_WGS),
              "graph_captures": sim.get_option(MPM_OPT_GRAPH_CAPTURES)}
     sim.close()
     print(f"[bench] batched: {len(ws)} sims, {npts} points, {args.ens_steps} steps in {wall:.2f} s", file=sys.stderr,

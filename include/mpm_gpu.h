@@ -162,7 +162,9 @@ int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
  *   MPM_OPT_AUTOTUNE       1: the next step call first times candidate launch shapes
  *                          (persistent particle-pass grid, grid-update grid) on its own
  *                          first steps and keeps the fastest; those steps count
- *   MPM_OPT_PASS_GRID      workgroups of the persistent particle passes (default 2048)
+ *   MPM_OPT_PASS_GRID      workgroups of the persistent G2P and fused particle passes
+ *                          (default 1024); setting it sets MPM_OPT_P2G_GRID too
+ *   MPM_OPT_P2G_GRID       workgroups of the persistent P2G pass (default 1536)
  *   MPM_OPT_GRID_WGS       workgroups of the grid update (default 2048)
  *   MPM_OPT_GRAPH_CAPTURES (get only) graphs captured so far */
 #define MPM_OPT_GRAPH 3
@@ -170,6 +172,7 @@ int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
 #define MPM_OPT_PASS_GRID 5
 #define MPM_OPT_GRID_WGS 6
 #define MPM_OPT_GRAPH_CAPTURES 7
+#define MPM_OPT_P2G_GRID 8
 int mpm_gpu_set_option(mpm_gpu_handle *h, int key, int value);
 int mpm_gpu_get_option(mpm_gpu_handle *h, int key, int *value);
 

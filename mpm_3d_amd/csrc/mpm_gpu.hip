@@ -217,7 +217,8 @@ struct mpm_gpu_handle {
     int par = 0;
     int buf = 0;  // which of the two particle buffers is current (toggled by every sort)
     int sort_interval = 16;
-    int pass_grid = 2048;  // persistent workgroups of the particle passes
+    int pass_grid = 1024;  // persistent workgroups of the G2P and fused particle passes
+    int p2g_grid = 1536;   // persistent workgroups of the P2G-only pass
     int grid_wgs = 2048;   // workgroups of the grid update
     bool autotune = false, tuned = false;
     bool use_graph = false;
@@ -371,7 +372,7 @@ int launch_sort(mpm_gpu_handle *h) {
 template <int DIM, class PT, bool G2P, bool P2G>
 void launch_particles(mpm_gpu_handle *h, int want_L) {
     KArgs A = kargs(h);
-    const dim3 grid(h->pass_grid);
+    const dim3 grid((!G2P && P2G) ? h->p2g_grid : h->pass_grid);
     if (h->lo.any_b1)
         hipLaunchKernelGGL((k_particles<DIM, G2P, P2G, true, PT>), grid, dim3(256), 0, h->stream, A, want_L);
     else
@@ -505,16 +506,18 @@ int graph_period(mpm_gpu_handle *h) {
 }
 
 // Launch-shape autotuner: times whole steps (HIP events) for candidate persistent-grid
-// sizes of the particle passes, then of the grid kernel, and keeps the fastest. The
+// sizes of the G2P (and fused) pass, then of the P2G pass, then of the grid kernel, and
+// keeps the fastest of each. The
 // steps it times are real steps of the run (results do not depend on launch shape beyond
 // atomic summation order). Returns the number of steps consumed.
 template <int DIM, class PT>
 int autotune(mpm_gpu_handle *h, int budget, int *used) {
     *used = 0;
     static const int pass_cand[] = {512, 1024, 2048, 4096, 8192};
+    static const int p2g_cand[] = {768, 1024, 1536, 2048, 4096};
     static const int grid_cand[] = {512, 2048, 8192};
     const int reps = 2;
-    const int need = reps * (5 + 3) + 1;
+    const int need = reps * (5 + 5 + 3) + 1;
     if (budget < need) return MPM_OK;  // too short a call to tune; tune on a later one
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -534,13 +537,21 @@ int autotune(mpm_gpu_handle *h, int budget, int *used) {
         return hipEventElapsedTime(ms, e0, e1) == hipSuccess ? MPM_OK : MPM_ERR_HIP;
     };
     float best = 1e30f, ms = 0;
-    int best_pass = h->pass_grid, best_grid = h->grid_wgs;
+    int best_pass = h->pass_grid, best_p2g = h->p2g_grid, best_grid = h->grid_wgs;
     for (int c : pass_cand) {
         h->pass_grid = c;
         if ((rc = measure(&ms))) break;
         if (ms < best) { best = ms; best_pass = c; }
     }
     h->pass_grid = best_pass;
+    best = 1e30f;
+    for (int c : p2g_cand) {
+        if (rc) break;
+        h->p2g_grid = c;
+        if ((rc = measure(&ms))) break;
+        if (ms < best) { best = ms; best_p2g = c; }
+    }
+    h->p2g_grid = best_p2g;
     best = 1e30f;
     for (int c : grid_cand) {
         if (rc) break;
@@ -991,9 +1002,12 @@ int mpm_gpu_set_option(mpm_gpu_handle *h, int key, int value) {
             h->tuned = false;
             return MPM_OK;
         case MPM_OPT_PASS_GRID:
+        case MPM_OPT_P2G_GRID:
         case MPM_OPT_GRID_WGS:
             if (value < 1 || value > (1 << 20)) return fail(MPM_ERR_ARG, "workgroup count out of range");
-            (key == MPM_OPT_PASS_GRID ? h->pass_grid : h->grid_wgs) = value;
+            if (key == MPM_OPT_PASS_GRID) h->pass_grid = h->p2g_grid = value;
+            else if (key == MPM_OPT_P2G_GRID) h->p2g_grid = value;
+            else h->grid_wgs = value;
             return MPM_OK;
         default:
             return fail(MPM_ERR_ARG, "unknown option %d", key);
@@ -1008,6 +1022,7 @@ int mpm_gpu_get_option(mpm_gpu_handle *h, int key, int *value) {
         case MPM_OPT_GRAPH: *value = h->use_graph; return MPM_OK;
         case MPM_OPT_AUTOTUNE: *value = h->autotune ? (h->tuned ? 2 : 1) : 0; return MPM_OK;
         case MPM_OPT_PASS_GRID: *value = h->pass_grid; return MPM_OK;
+        case MPM_OPT_P2G_GRID: *value = h->p2g_grid; return MPM_OK;
         case MPM_OPT_GRID_WGS: *value = h->grid_wgs; return MPM_OK;
         case MPM_OPT_GRAPH_CAPTURES: *value = h->graph_captures; return MPM_OK;
         default: return fail(MPM_ERR_ARG, "unknown option %d", key);

@@ -226,6 +226,14 @@ __device__ __forceinline__ void fetch_node(const KArgs &A, const SimDev &s, long
 #ifndef MPM_PF_P2G
 #define MPM_PF_P2G 0
 #endif
+// Lane order of the P2G pass inside a wavefront: 0 slot order; 1 and 2 place the point of
+// in-block dual cell (x, y, z) on lane x + 4y + 16z (1) or x + 4z + 16y (2) (2D: x + 8y,
+// y + 8x) when the
+// wavefront's points sit in distinct dual cells (else slot order), so that the lanes
+// served together by the LDS hit distinct banks.
+#ifndef MPM_P2G_ORDER
+#define MPM_P2G_ORDER 2
+#endif
 template <bool DO_G2P, bool DO_P2G, bool FP32> struct PassTune {
     static constexpr int waves = (DO_G2P && DO_P2G) ? 2  // the fused pass
                                  : DO_G2P ? (FP32 ? MPM_WPE_G2P32 : MPM_WPE_G2P64)
@@ -243,7 +251,8 @@ template <bool DO_G2P, bool DO_P2G, bool FP32> struct PassTune {
 template <int DIM, bool DO_G2P, bool DO_P2G, bool NRM, class PT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PassTune<DO_G2P, DO_P2G, PT::FP32>::waves)))
 void k_particles(KArgs A, int want_L) {
-    constexpr bool PREFETCH = PassTune<DO_G2P, DO_P2G, PT::FP32>::prefetch;
+    constexpr int ORDER = (!DO_G2P && DO_P2G) ? MPM_P2G_ORDER : 0;
+    constexpr bool PREFETCH = PassTune<DO_G2P, DO_P2G, PT::FP32>::prefetch && ORDER == 0;
     using R = typename PT::R;
     using GO = typename PT::GO;
     using St = typename PT::template State<DIM>;
@@ -261,6 +270,7 @@ void k_particles(KArgs A, int want_L) {
     __shared__ float laccf[DO_P2G && FF32 ? DIM * PSTR : 1];
     __shared__ GO gpat[DO_G2P ? NOUT * PSTR : 1];
     __shared__ int bb[2 * DIM];
+    __shared__ int wslot[ORDER ? 256 : 1];  // per wavefront: the lane that holds each dual cell
     const int tid = threadIdx.x;
     const int nent = *A.npblk;
     const R sscale = (R)1 / (R)(1 << DIM);
@@ -317,8 +327,42 @@ void k_particles(KArgs A, int want_L) {
             an = A.act[start + tid] != 0;
             if (an) PT::template load<DIM>(A, start + tid, qn, DO_P2G);
         }
+        const int lane = tid & 63;
 #pragma unroll 1
-        for (int k = tid; k < count; k += blockDim.x) {
+        for (int kw = ORDER ? tid - lane : tid; kw < count; kw += blockDim.x) {  // ORDER: wavefront-uniform
+            int k = kw;
+            if (ORDER) {
+                // key: the point's dual cell inside the particle block (2 or 3 bits per dimension)
+                constexpr int KB = Cfg<DIM>::PB == 4 ? 2 : 3;
+                static_assert(KB * DIM == 6, "one key per lane of a wavefront");
+                k = kw + lane;
+                int key = -1;
+                bool clash = false;
+                if (k < count && A.act[start + k]) {
+                    int dual[DIM], oct;
+                    if (PT::template dual_cell<DIM>(A, s, start + k, dual, oct)) {
+                        const int m = (1 << KB) - 1;
+                        const int a0 = dual[0] & m, a1 = dual[1] & m, a2 = DIM == 3 ? dual[DIM - 1] & m : 0;
+                        if (DIM == 3) key = ORDER == 1 ? a0 | a1 << KB | a2 << (2 * KB) : a0 | a2 << KB | a1 << (2 * KB);
+                        else key = ORDER == 1 ? a0 | a1 << KB : a1 | a0 << KB;  // 64 keys: KB bits per dimension
+                    } else {
+                        clash = true;  // leaves the domain: handled in slot order below
+                    }
+                }
+                int *ws = &wslot[(tid >> 6) * 64];
+                ws[lane] = -1;
+                __builtin_amdgcn_wave_barrier();
+                if (key >= 0) ws[key] = lane;
+                __builtin_amdgcn_wave_barrier();
+                if (key >= 0 && ws[key] != lane) clash = true;
+                const bool any_clash = __ballot(clash) != 0ull;
+                if (!any_clash) {
+                    const int src = ws[lane];
+                    k = src >= 0 ? kw + src : count;
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (k >= count) continue;
+            }
             const long long p = start + k;
             St q;
             if (PREFETCH) {

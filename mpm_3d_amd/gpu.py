@@ -36,6 +36,7 @@ MPM_OPT_AUTOTUNE = 4
 MPM_OPT_PASS_GRID = 5
 MPM_OPT_GRID_WGS = 6
 MPM_OPT_GRAPH_CAPTURES = 7
+MPM_OPT_P2G_GRID = 8
 
 
 class MpmGpuError(RuntimeError):

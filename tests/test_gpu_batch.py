@@ -12,7 +12,7 @@ import pytest
 from mpm_3d_amd import serve as S
 from mpm_3d_amd import workloads as W
 from mpm_3d_amd.gpu import (MPM_OPT_AUTOTUNE, MPM_OPT_GRAPH, MPM_OPT_GRAPH_CAPTURES, MPM_OPT_GRID_WGS,
-                            MPM_OPT_PASS_GRID, MPM_PREC_FP32, MPM_PREC_FP64, GpuSim)
+                            MPM_OPT_P2G_GRID, MPM_OPT_PASS_GRID, MPM_PREC_FP32, MPM_PREC_FP64, GpuSim)
 from tests._harness import oracle_of
 
 pytestmark = pytest.mark.gpu
@@ -80,6 +80,7 @@ def test_autotune_keeps_results():
     b.step_batch(dts, 60)
     assert b.get_option(MPM_OPT_AUTOTUNE) == 2
     assert b.get_option(MPM_OPT_PASS_GRID) in (512, 1024, 2048, 4096, 8192)
+    assert b.get_option(MPM_OPT_P2G_GRID) in (768, 1024, 1536, 2048, 4096)
     assert b.get_option(MPM_OPT_GRID_WGS) in (512, 2048, 8192)
     for i, w in enumerate(ws):
         ex, ev = _err(w, a.points(i), b.points(i))
