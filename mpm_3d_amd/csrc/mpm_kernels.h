@@ -234,6 +234,34 @@ __device__ __forceinline__ void fetch_node(const KArgs &A, const SimDev &s, long
 #ifndef MPM_P2G_ORDER
 #define MPM_P2G_ORDER 2
 #endif
+// Lane of the P2G lane order for a point of in-block dual cell `dual` (KB = 2 or 3 bits per
+// dimension, 64 keys).
+template <int DIM, int ORDER>
+__device__ __forceinline__ int lane_key(const int *dual) {
+    constexpr int KB = Cfg<DIM>::PB == 4 ? 2 : 3;
+    static_assert(KB * DIM == 6, "one key per lane of a wavefront");
+    constexpr int m = (1 << KB) - 1;
+    const int a0 = dual[0] & m, a1 = dual[1] & m, a2 = DIM == 3 ? dual[DIM - 1] & m : 0;
+    if (DIM == 3) return ORDER == 1 ? a0 | a1 << KB | a2 << (2 * KB) : a0 | a2 << KB | a1 << (2 * KB);
+    return ORDER == 1 ? a0 | a1 << KB : a1 | a0 << KB;
+}
+
+// Called by the whole wavefront: lane L's point has lane-order key `key` (-1: no point),
+// `clash` marks a point that cannot take part. Returns, for every lane, the lane whose
+// point it takes (-1: none), or its own lane for every lane when two keys coincide or any
+// lane clashed. ws: 64 ints of wavefront-private LDS.
+__device__ __forceinline__ int lane_source(int *ws, int lane, int key, bool clash) {
+    ws[lane] = -1;
+    __builtin_amdgcn_wave_barrier();
+    if (key >= 0) ws[key] = lane;
+    __builtin_amdgcn_wave_barrier();
+    if (key >= 0 && ws[key] != lane) clash = true;
+    const bool any_clash = __ballot(clash) != 0ull;
+    const int src = any_clash ? lane : ws[lane];
+    __builtin_amdgcn_wave_barrier();
+    return src;
+}
+
 template <bool DO_G2P, bool DO_P2G, bool FP32> struct PassTune {
     static constexpr int waves = (DO_G2P && DO_P2G) ? 2  // the fused pass
                                  : DO_G2P ? (FP32 ? MPM_WPE_G2P32 : MPM_WPE_G2P64)
@@ -252,7 +280,7 @@ template <int DIM, bool DO_G2P, bool DO_P2G, bool NRM, class PT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PassTune<DO_G2P, DO_P2G, PT::FP32>::waves)))
 void k_particles(KArgs A, int want_L) {
     constexpr int ORDER = (!DO_G2P && DO_P2G) ? MPM_P2G_ORDER : 0;
-    constexpr bool PREFETCH = PassTune<DO_G2P, DO_P2G, PT::FP32>::prefetch && ORDER == 0;
+    constexpr bool PREFETCH = PassTune<DO_G2P, DO_P2G, PT::FP32>::prefetch;
     using R = typename PT::R;
     using GO = typename PT::GO;
     using St = typename PT::template State<DIM>;
@@ -320,7 +348,9 @@ void k_particles(KArgs A, int want_L) {
             thi[d] = -0x7fffffff;
         }
         // with PREFETCH the next point's state is loaded before this one is processed, so
-        // its loads are in flight during the LDS work (different slots: no ordering hazard)
+        // its loads are in flight during the LDS work (different slots: no ordering hazard).
+        // (Carrying the prefetch over to the next particle block costs the G2P pass more
+        // registers than it saves in latency.)
         St qn;
         bool an = false;
         if (PREFETCH && tid < count) {
@@ -330,43 +360,26 @@ void k_particles(KArgs A, int want_L) {
         const int lane = tid & 63;
 #pragma unroll 1
         for (int kw = ORDER ? tid - lane : tid; kw < count; kw += blockDim.x) {  // ORDER: wavefront-uniform
-            int k = kw;
-            if (ORDER) {
-                // key: the point's dual cell inside the particle block (2 or 3 bits per dimension)
-                constexpr int KB = Cfg<DIM>::PB == 4 ? 2 : 3;
-                static_assert(KB * DIM == 6, "one key per lane of a wavefront");
-                k = kw + lane;
+            int k = ORDER ? kw + lane : kw;
+            St q;
+            bool a;
+            if (ORDER && !PT::FP32) {
+                // FP64 state: choose the lane order from the dual cells first, then load each
+                // lane's point (moving 15 doubles across lanes costs more registers than the
+                // second round trip)
                 int key = -1;
                 bool clash = false;
                 if (k < count && A.act[start + k]) {
                     int dual[DIM], oct;
-                    if (PT::template dual_cell<DIM>(A, s, start + k, dual, oct)) {
-                        const int m = (1 << KB) - 1;
-                        const int a0 = dual[0] & m, a1 = dual[1] & m, a2 = DIM == 3 ? dual[DIM - 1] & m : 0;
-                        if (DIM == 3) key = ORDER == 1 ? a0 | a1 << KB | a2 << (2 * KB) : a0 | a2 << KB | a1 << (2 * KB);
-                        else key = ORDER == 1 ? a0 | a1 << KB : a1 | a0 << KB;  // 64 keys: KB bits per dimension
-                    } else {
-                        clash = true;  // leaves the domain: handled in slot order below
-                    }
+                    if (PT::template dual_cell<DIM>(A, s, start + k, dual, oct)) key = lane_key<DIM, ORDER>(dual);
+                    else clash = true;
                 }
-                int *ws = &wslot[(tid >> 6) * 64];
-                ws[lane] = -1;
-                __builtin_amdgcn_wave_barrier();
-                if (key >= 0) ws[key] = lane;
-                __builtin_amdgcn_wave_barrier();
-                if (key >= 0 && ws[key] != lane) clash = true;
-                const bool any_clash = __ballot(clash) != 0ull;
-                if (!any_clash) {
-                    const int src = ws[lane];
-                    k = src >= 0 ? kw + src : count;
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (k >= count) continue;
-            }
-            const long long p = start + k;
-            St q;
-            if (PREFETCH) {
-                const bool a = an;
+                const int src = lane_source(&wslot[(tid >> 6) * 64], lane, key, clash);
+                if (src != lane) k = src >= 0 ? kw + src : count;
+                a = k < count && A.act[start + k] != 0;
+                if (a) PT::template load<DIM>(A, start + k, q, DO_P2G);
+            } else if (PREFETCH) {
+                a = an;
                 q = qn;
                 const int kn = k + blockDim.x;
                 an = false;
@@ -374,11 +387,29 @@ void k_particles(KArgs A, int want_L) {
                     an = A.act[start + kn] != 0;
                     if (an) PT::template load<DIM>(A, start + kn, qn, DO_P2G);
                 }
-                if (!a) continue;
             } else {
-                if (!A.act[p]) continue;
-                PT::template load<DIM>(A, p, q, DO_P2G);
+                a = k < count && A.act[start + k] != 0;
+                if (a) PT::template load<DIM>(A, start + k, q, DO_P2G);
             }
+            if (ORDER && PT::FP32) {
+                // FP32 state: load in slot order, then move the loaded points across the
+                // wavefront into the lane order
+                int key = -1;
+                bool clash = false;
+                if (a) {
+                    int dual[DIM];
+                    if (PT::template dual_of<DIM>(s, q, dual)) key = lane_key<DIM, ORDER>(dual);
+                    else clash = true;
+                }
+                const int src = lane_source(&wslot[(tid >> 6) * 64], lane, key, clash);
+                if (__ballot(src != lane) != 0ull) {  // every lane takes part in the shuffle
+                    PT::template shuffle<DIM>(q, src < 0 ? lane : src);
+                    a = src >= 0;
+                    k = kw + (src < 0 ? lane : src);
+                }
+            }
+            if (!a) continue;
+            const long long p = start + k;
             if (DO_G2P) {
                 Win<DIM, R> w;
                 bool cpdi;
