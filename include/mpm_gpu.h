@@ -140,6 +140,25 @@ int mpm_gpu_download(mpm_gpu_handle *h, int body, double *x, double *v, double *
  * Replaces reads of Points::u and Points::L. */
 int mpm_gpu_download_aux(mpm_gpu_handle *h, int body, double *u, double *L);
 
+/* Node fields of one body after the last step call, in the reference's node numbering
+ * (CartesianLinear: n = i + (N0+1)(j + (N1+1)k), (N0+1)(N1+1)[(N2+1)] nodes for the body's
+ * sim), vectors as dim planes of that many doubles. m (mass), mxt (momentum after
+ * moveGrid) and f (force after contact and boundary) need MPM_OPT_NODE_RECORD on during
+ * that call; xt and dxt (velocity and its increment, moveGrid) are always available. Nodes
+ * the last grid update did not reach (no point near them) report 0, as the reference's
+ * massless nodes. Any pointer may be NULL. MPM_ERR_STATE before the first step after an
+ * upload or load. Replaces reads of Nodes::m, mx_t, f, x_t, diff_x_t (mpm_objects.hpp:164-167),
+ * which DefaultNodes::writeFrame serializes (default_nodes.cpp:68-82). */
+int mpm_gpu_download_nodes(mpm_gpu_handle *h, int body, double *m, double *mxt, double *f, double *xt,
+                           double *dxt);
+
+/* ContactHuang's contact_normal and contact_force (contacts.hpp:56) of contact `contact`
+ * (global index, create order) over its sim's nodes, numbered as above, after the last step
+ * call (recorded under MPM_OPT_NODE_RECORD): the normal is body 1's normalised mass gradient
+ * (NaN where it is zero, as the reference's 0/0), the force the contact force on body 1 where
+ * both bodies have mass (contact_huang.cpp:80-197), else 0. Either pointer may be NULL. */
+int mpm_gpu_download_contact(mpm_gpu_handle *h, int contact, double *normal, double *force);
+
 /* Per-point connectivity of the current state, original order: code (MPM_CONN_*) and the
  * element of each of the 2^dim GIMP corners (-1 where none; CPDI-off points report their
  * own element in slot 0). Replaces DefaultPoints::generateMap's element lookups
@@ -166,13 +185,19 @@ int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
  *                          (default 1024); setting it sets MPM_OPT_P2G_GRID too
  *   MPM_OPT_P2G_GRID       workgroups of the persistent P2G pass (default 1536)
  *   MPM_OPT_GRID_WGS       workgroups of the grid update (default 2048)
- *   MPM_OPT_GRAPH_CAPTURES (get only) graphs captured so far */
+ *   MPM_OPT_GRAPH_CAPTURES (get only) graphs captured so far
+ *   MPM_OPT_NODE_RECORD    1: every grid update also records nodal mass, momentum and force
+ *                          and the contacts' normals and forces, for mpm_gpu_download_nodes
+ *                          and mpm_gpu_download_contact ((1 + 2 dim) doubles per node plus
+ *                          2 dim per node of each contact's sim, allocated on first use and
+ *                          not counted by mpm_gpu_footprint); graph replay is off meanwhile */
 #define MPM_OPT_GRAPH 3
 #define MPM_OPT_AUTOTUNE 4
 #define MPM_OPT_PASS_GRID 5
 #define MPM_OPT_GRID_WGS 6
 #define MPM_OPT_GRAPH_CAPTURES 7
 #define MPM_OPT_P2G_GRID 8
+#define MPM_OPT_NODE_RECORD 9
 int mpm_gpu_set_option(mpm_gpu_handle *h, int key, int value);
 int mpm_gpu_get_option(mpm_gpu_handle *h, int key, int *value);
 

@@ -65,8 +65,9 @@ struct BodyDev {
 };
 
 struct ContactDev {
-    int b1, b2;  // sim-local body indices
+    int b1, b2;     // sim-local body indices
     double mu;
+    long long rec;  // first element of the contact's record planes in KArgs::crec
 };
 
 struct PBlk {  // a run of sorted particles sharing one particle block
@@ -169,6 +170,11 @@ struct KArgs {
     const int *npblk;
     int par;
     double *Lout;  // 9 planes x np, slot order (optional)
+    // node record of the grid update (optional): m, m v after moveGrid, f after contact and
+    // boundary, 1 + 2 DIM planes x nt; per contact slot: normal and force, 2 DIM planes over
+    // its sim's node storage, at ContactDev::rec
+    double *rec;
+    double *crec;
     unsigned long long *stats;  // [0] active nodes of the last grid update
 };
 

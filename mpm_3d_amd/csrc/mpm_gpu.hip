@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <limits>
 #include <vector>
 
 #include "mpm_gpu.h"
@@ -52,6 +53,7 @@ struct Layout {  // sizes derived from the descriptors, shared by create and foo
     std::vector<BodyDev> bodies;
     std::vector<ContactDev> contacts;
     long long np = 0, nt = 0, nblk_total = 0, nkeys = 0, npblk_total = 0;
+    long long crec_total = 0;  // contact record elements (ContactDev::rec), allocated on demand
     bool any_b1 = false;
 };
 
@@ -118,7 +120,8 @@ int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
                 return fail(MPM_ERR_ARG, "sim %d contact %d: bodies must be two distinct sim-local indices", i, c);
             is_b1[cd.body1] = 1;
             Lo.any_b1 = true;
-            ContactDev C{cd.body1, cd.body2, cd.mu};
+            ContactDev C{cd.body1, cd.body2, cd.mu, Lo.crec_total};
+            Lo.crec_total += 2LL * Lo.dim * nblk * 64;
             Lo.contacts.push_back(C);
         }
         for (int b = 0; b < sd[i].nbody; b++) {
@@ -212,6 +215,11 @@ struct mpm_gpu_handle {
     int *npblk = nullptr;
     double *Lout = nullptr;
     unsigned long long *stats = nullptr;
+    bool record = false;      // MPM_OPT_NODE_RECORD
+    double *rec = nullptr;    // node record planes (KArgs::rec)
+    double *crec = nullptr;   // contact record planes (KArgs::crec)
+    bool recorded = false;    // the last grid update wrote the record
+    bool grid_valid = false;  // a step ran since the last upload / load / clear (node downloads)
     double *stage = nullptr;  // download staging
     long long stage_elems = 0;
     int par = 0;
@@ -269,6 +277,8 @@ KArgs kargs(mpm_gpu_handle *h) {
     A.par = h->par;
     A.Lout = h->Lout;
     A.stats = h->stats;
+    A.rec = nullptr;  // set by launch_grid when recording
+    A.crec = nullptr;
     return A;
 }
 
@@ -384,6 +394,11 @@ void launch_grid(mpm_gpu_handle *h, bool want_stats) {
     KArgs Ag = kargs(h);
     if (want_stats) hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(64), 0, h->stream, h->stats, 4);
     else Ag.stats = nullptr;
+    if (h->record) {
+        Ag.rec = h->rec;
+        Ag.crec = h->crec;
+    }
+    h->recorded = h->record;
     hipLaunchKernelGGL((k_grid<DIM, typename PT::GO>), dim3(h->grid_wgs), dim3(256), 0, h->stream, Ag);
 }
 
@@ -577,7 +592,8 @@ int launch_step(mpm_gpu_handle *h, int nsteps) {
         if ((rc = autotune<DIM, PT>(h, nsteps - 1, &used))) return rc;
         k = used;
     }
-    const bool graphs = h->use_graph && !h->timing && h->sort_interval > 0;
+    // (no graph replay while recording: the record pointers would be baked into the graph)
+    const bool graphs = h->use_graph && !h->timing && h->sort_interval > 0 && !h->record;
     while (k < nsteps) {
         const bool last = k == nsteps - 1;
         if (graphs && h->sorted && h->since_sort >= h->sort_interval && nsteps - 1 - k >= h->sort_interval) {
@@ -832,6 +848,7 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
             swap_buffers(h);
             h->x0[body].assign(x, x + (long long)dim * n);
             h->uploaded[body] = 1;
+            h->grid_valid = false;
             h->sorted = false;
             return MPM_OK;
         }
@@ -866,6 +883,7 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
     }
     h->x0[body].assign(x, x + (long long)dim * n);
     h->uploaded[body] = 1;
+    h->grid_valid = false;
     h->sorted = false;
     return MPM_OK;
 }
@@ -944,10 +962,14 @@ int mpm_gpu_load_sim(mpm_gpu_handle *h, int sim, const mpm_sim_desc *d) {
             h->count[gb] = 0;
         }
         h->uploaded[gb] = h->count[gb] == 0;
+        h->grid_valid = false;
         h->x0[gb].clear();
     }
     for (int c = 0; c < d->ncontact; c++) {
-        h->lo.contacts[s.contact0 + c] = ContactDev{d->contacts[c].body1, d->contacts[c].body2, d->contacts[c].mu};
+        ContactDev &C = h->lo.contacts[s.contact0 + c];  // the slot keeps its record planes (C.rec)
+        C.b1 = d->contacts[c].body1;
+        C.b2 = d->contacts[c].body2;
+        C.mu = d->contacts[c].mu;
         h->lo.bodies[s.body0 + d->contacts[c].body1].is_b1 = 1;
     }
     HIP_TRY(hipMemcpyAsync(h->d_bodies, h->lo.bodies.data(), sizeof(BodyDev) * h->lo.bodies.size(),
@@ -971,6 +993,7 @@ int mpm_gpu_clear_sim(mpm_gpu_handle *h, int sim) {
         h->count[s.body0 + b] = 0;
         h->uploaded[s.body0 + b] = 1;
     }
+    h->grid_valid = false;
     s.ncontact = 0;
     h->sims_dirty = true;
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1001,6 +1024,19 @@ int mpm_gpu_set_option(mpm_gpu_handle *h, int key, int value) {
             h->autotune = value != 0;
             h->tuned = false;
             return MPM_OK;
+        case MPM_OPT_NODE_RECORD:
+            if (value != 0 && !h->rec) {
+                const long long nr = (1LL + 2 * h->lo.dim) * std::max(1ll, h->lo.nt);
+                HIP_TRY(hipMalloc(&h->rec, sizeof(double) * nr));
+                HIP_TRY(hipMemsetAsync(h->rec, 0, sizeof(double) * nr, h->stream));
+                if (h->lo.crec_total > 0) {
+                    HIP_TRY(hipMalloc(&h->crec, sizeof(double) * h->lo.crec_total));
+                    HIP_TRY(hipMemsetAsync(h->crec, 0, sizeof(double) * h->lo.crec_total, h->stream));
+                }
+                HIP_TRY(hipStreamSynchronize(h->stream));
+            }
+            h->record = value != 0;
+            return MPM_OK;
         case MPM_OPT_PASS_GRID:
         case MPM_OPT_P2G_GRID:
         case MPM_OPT_GRID_WGS:
@@ -1023,6 +1059,7 @@ int mpm_gpu_get_option(mpm_gpu_handle *h, int key, int *value) {
         case MPM_OPT_AUTOTUNE: *value = h->autotune ? (h->tuned ? 2 : 1) : 0; return MPM_OK;
         case MPM_OPT_PASS_GRID: *value = h->pass_grid; return MPM_OK;
         case MPM_OPT_P2G_GRID: *value = h->p2g_grid; return MPM_OK;
+        case MPM_OPT_NODE_RECORD: *value = h->record; return MPM_OK;
         case MPM_OPT_GRID_WGS: *value = h->grid_wgs; return MPM_OK;
         case MPM_OPT_GRAPH_CAPTURES: *value = h->graph_captures; return MPM_OK;
         default: return fail(MPM_ERR_ARG, "unknown option %d", key);
@@ -1078,6 +1115,7 @@ static int run_steps(mpm_gpu_handle *h, const double *dt, int nsim, int nsteps) 
         if (r2) return r2;
     }
     HIP_TRY(hipMemcpy(&h->last_active_nodes, h->stats, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    h->grid_valid = true;
     return MPM_OK;
 }
 
@@ -1189,6 +1227,107 @@ int mpm_gpu_download_aux(mpm_gpu_handle *h, int body, double *u, double *L) {
                                    hipMemcpyDeviceToHost, h->stream));
         HIP_TRY(hipStreamSynchronize(h->stream));
     }
+    return MPM_OK;
+}
+
+}  // extern "C"
+
+// node-block list of the last grid update: list par^1 (split and fused steps flip par
+// after the grid update; the list stays intact until the next step's P2G fills the other)
+static int last_grid_list(mpm_gpu_handle *h, const int **lst, int *cnt) {
+    const int lp = h->par ^ 1;
+    HIP_TRY(hipMemcpy(cnt, h->ncount + lp, sizeof(int), hipMemcpyDeviceToHost));
+    *lst = h->nlist + (long long)lp * h->lo.nblk_total;
+    return MPM_OK;
+}
+
+static long long sim_nodes(const SimDev &s, int dim) {
+    long long nn = 1;
+    for (int d = 0; d < dim; d++) nn *= s.N[d] + 1;
+    return nn;
+}
+
+template <int DIM, class GO>
+static void launch_nodes_out(mpm_gpu_handle *h, int si, long long grid_base, const int *lst, int cnt, int with_rec,
+                             int contact, double *out, long long nn) {
+    if (cnt <= 0) return;
+    KArgs A = kargs(h);
+    A.rec = h->rec;  // kargs leaves the record unset (launch_grid sets it per update)
+    A.crec = h->crec;
+    hipLaunchKernelGGL((k_nodes_out<DIM, GO>), dim3((unsigned)ceil_div(cnt, 4)), dim3(256), 0, h->stream, A, si,
+                       grid_base, lst, cnt, with_rec, contact, out, nn);
+}
+
+extern "C" {
+
+int mpm_gpu_download_nodes(mpm_gpu_handle *h, int body, double *m, double *mxt, double *f, double *xt, double *dxt) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    if (body < 0 || body >= (int)h->lo.bodies.size()) return fail(MPM_ERR_ARG, "body %d out of range", body);
+    if (!h->grid_valid) return fail(MPM_ERR_STATE, "no step since the last upload or load: no node values");
+    const bool want_rec = m || mxt || f;
+    if (want_rec && (!h->recorded || !h->rec))
+        return fail(MPM_ERR_STATE, "m, mxt and f need MPM_OPT_NODE_RECORD on during the last step call");
+    const BodyDev &B = h->lo.bodies[body];
+    const SimDev &s = h->lo.sims[B.sim];
+    const int dim = h->lo.dim;
+    const long long nn = sim_nodes(s, dim);
+    const int with_rec = want_rec ? 1 : 0;
+    const long long planes = (with_rec ? 1 + 2 * dim : 0) + 2 * dim;
+    int rc = ensure_stage(h, planes * nn);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(h->stage, 0, sizeof(double) * planes * nn, h->stream));
+    const int *lst = nullptr;
+    int cnt = 0;
+    if ((rc = last_grid_list(h, &lst, &cnt))) return rc;
+    const bool f32 = h->precision == MPM_PREC_FP32;
+    if (dim == 3) {
+        if (f32) launch_nodes_out<3, float>(h, B.sim, B.grid_base, lst, cnt, with_rec, -1, h->stage, nn);
+        else launch_nodes_out<3, double>(h, B.sim, B.grid_base, lst, cnt, with_rec, -1, h->stage, nn);
+    } else {
+        if (f32) launch_nodes_out<2, float>(h, B.sim, B.grid_base, lst, cnt, with_rec, -1, h->stage, nn);
+        else launch_nodes_out<2, double>(h, B.sim, B.grid_base, lst, cnt, with_rec, -1, h->stage, nn);
+    }
+    HIP_TRY(hipGetLastError());
+    const size_t vb = sizeof(double) * dim * nn;
+    const long long o = with_rec ? 1 + 2 * dim : 0;
+    if (m) HIP_TRY(hipMemcpyAsync(m, h->stage, sizeof(double) * nn, hipMemcpyDeviceToHost, h->stream));
+    if (mxt) HIP_TRY(hipMemcpyAsync(mxt, h->stage + nn, vb, hipMemcpyDeviceToHost, h->stream));
+    if (f) HIP_TRY(hipMemcpyAsync(f, h->stage + (1 + dim) * nn, vb, hipMemcpyDeviceToHost, h->stream));
+    if (xt) HIP_TRY(hipMemcpyAsync(xt, h->stage + o * nn, vb, hipMemcpyDeviceToHost, h->stream));
+    if (dxt) HIP_TRY(hipMemcpyAsync(dxt, h->stage + (o + dim) * nn, vb, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return MPM_OK;
+}
+
+int mpm_gpu_download_contact(mpm_gpu_handle *h, int contact, double *normal, double *force) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    if (contact < 0 || contact >= (int)h->lo.contacts.size()) return fail(MPM_ERR_ARG, "contact %d out of range", contact);
+    int si = 0;
+    while (si + 1 < (int)h->lo.sims.size() && h->lo.sims[si + 1].contact0 <= contact) si++;
+    const SimDev &s = h->lo.sims[si];
+    if (contact - s.contact0 >= s.ncontact) return fail(MPM_ERR_ARG, "contact slot %d is not in use", contact);
+    if (!h->grid_valid) return fail(MPM_ERR_STATE, "no step since the last upload or load: no node values");
+    if (!h->recorded || !h->crec)
+        return fail(MPM_ERR_STATE, "contact values need MPM_OPT_NODE_RECORD on during the last step call");
+    const int dim = h->lo.dim;
+    const long long nn = sim_nodes(s, dim);
+    int rc = ensure_stage(h, 2LL * dim * nn);
+    if (rc) return rc;
+    // nodes the last update did not reach: normal 0/0 (no mass gradient), force 0
+    const double qnan = std::numeric_limits<double>::quiet_NaN();
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)ceil_div(dim * nn, 256)), dim3(256), 0, h->stream, h->stage,
+                       (long long)dim * nn, qnan);
+    HIP_TRY(hipMemsetAsync(h->stage + dim * nn, 0, sizeof(double) * dim * nn, h->stream));
+    const int *lst = nullptr;
+    int cnt = 0;
+    if ((rc = last_grid_list(h, &lst, &cnt))) return rc;
+    if (dim == 3) launch_nodes_out<3, double>(h, si, 0, lst, cnt, 0, contact, h->stage, nn);
+    else launch_nodes_out<2, double>(h, si, 0, lst, cnt, 0, contact, h->stage, nn);
+    HIP_TRY(hipGetLastError());
+    const size_t vb = sizeof(double) * dim * nn;
+    if (normal) HIP_TRY(hipMemcpyAsync(normal, h->stage, vb, hipMemcpyDeviceToHost, h->stream));
+    if (force) HIP_TRY(hipMemcpyAsync(force, h->stage + dim * nn, vb, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
     return MPM_OK;
 }
 
@@ -1365,7 +1504,7 @@ int mpm_gpu_destroy(mpm_gpu_handle *h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void *ptrs[] = {h->d_sims, h->d_bodies, h->d_contacts, h->P, h->P2, h->Pw, h->Pw2, h->act, h->act2, h->pid,
                     h->pid2, h->pkey, h->prank, h->acc, h->out, h->out32, h->nflag, h->nlist, h->ncount, h->bcount, h->boff,
-                    h->plist, h->npblk, h->Lout, h->stats, h->stage};
+                    h->plist, h->npblk, h->Lout, h->stats, h->stage, h->rec, h->crec};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (unsigned *p : h->scan_tmp)

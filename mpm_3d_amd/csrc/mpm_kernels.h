@@ -774,6 +774,22 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
                         for (int d = 0; d < DIM; d++) { p2[d] = p[b][d]; f2[d] = f[b][d]; }
                     }
                 }
+                // record: contact_normal at every node (generateRules, contact_huang.cpp:80-99:
+                // 0/0 where body 1 has no mass gradient), contact_force 0 unless set below
+                double *cr = nullptr;
+                const long long S = s.nblk * 64;
+                if (A.crec) {
+                    cr = A.crec + C.rec;
+                    double nn = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) nn += n1[d] * n1[d];
+                    const double rn = 1.0 / sqrt(nn);
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        cr[d * S + addr] = n1[d] * rn;
+                        cr[(DIM + d) * S + addr] = 0.0;
+                    }
+                }
                 if (!(m1 > 0.0 && m2 > 0.0)) continue;
                 double nn = 0.0;
 #pragma unroll
@@ -826,6 +842,10 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
                         if (b == C.b2) f[b][d] = f[b][d] - fc;
                     }
                 }
+                if (cr) {
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) cr[(DIM + d) * S + addr] = nrm[d] * fnc + s1[d] * fsc;
+                }
             }
             // the body's boundary, then moveGrid
 #pragma unroll
@@ -876,11 +896,16 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
                 const long long gi = B.grid_base + addr;
                 const bool has = m[b] > 0.0;
                 const double rm = has ? 1.0 / m[b] : 0.0;
+                if (A.rec) A.rec[gi] = m[b];
 #pragma unroll
                 for (int d = 0; d < DIM; d++) {
                     const double pn = p[b][d] + f[b][d] * dt;
                     outp[(G::OV + d) * A.nt + gi] = (GO)(has ? pn * rm : 0.0);
                     outp[(G::ODV + d) * A.nt + gi] = (GO)(has ? (f[b][d] * dt) * rm : 0.0);
+                    if (A.rec) {
+                        A.rec[(1 + d) * A.nt + gi] = pn;
+                        A.rec[(1 + DIM + d) * A.nt + gi] = f[b][d];
+                    }
                 }
             }
         }
@@ -891,6 +916,67 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
         unsigned long long tot = active_nodes;
         for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
         if (lane == 0 && tot) atomicAdd(&A.stats[0], tot);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Node fields of the last grid update in the reference's node order (CartesianLinear:
+// n = i + (N0+1)(j + (N1+1) k)), from the node blocks that update processed (list `lst`,
+// `cnt` entries, global block ids); nodes outside them keep what the caller filled in.
+// contact < 0: body planes into out, [m, mxt DIM, f DIM] (with_rec) then [xt DIM, dxt DIM]
+// from the grid output, nn per plane, body storage at grid_base. contact >= 0: [normal DIM,
+// force DIM] of that contact slot's record. One wavefront per list entry.
+template <int DIM, class GO>
+__global__ __launch_bounds__(256) void k_nodes_out(KArgs A, int si, long long grid_base, const int *lst, int cnt,
+                                                    int with_rec, int contact, double *out, long long nn) {
+    const GO *outp = nullptr;
+    if constexpr (sizeof(GO) == 8) outp = reinterpret_cast<const GO *>(A.out);
+    else outp = reinterpret_cast<const GO *>(A.out32);
+    using G = GF<DIM>;
+    constexpr int NB = Cfg<DIM>::NB, L = Cfg<DIM>::LNB;
+    const SimDev &s = A.sims[si];
+    const int lane = threadIdx.x & 63;
+    const int bi = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (bi >= cnt) return;
+    const long long lb = lst[bi] - s.nblk_off;
+    if (lb < 0 || lb >= s.nblk) return;  // another sim's block
+    int ijk[3] = {0, 0, 0};
+    long long t = lb;
+    const int bx = (int)(t % s.NBn[0]);
+    t /= s.NBn[0];
+    ijk[0] = (bx << L) + (lane & (NB - 1)) - NSHIFT;
+    if (DIM == 3) {
+        ijk[1] = ((int)(t % s.NBn[1]) << L) + ((lane >> L) & (NB - 1)) - NSHIFT;
+        ijk[2] = ((int)(t / s.NBn[1]) << L) + (lane >> (2 * L)) - NSHIFT;
+    } else {
+        ijk[1] = ((int)t << L) + (lane >> L) - NSHIFT;
+    }
+    bool valid = true;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) valid = valid && ijk[d] >= 0 && ijk[d] <= s.N[d];
+    if (!valid) return;
+    long long nid = ijk[DIM - 1];
+#pragma unroll
+    for (int d = DIM - 2; d >= 0; d--) nid = nid * (s.N[d] + 1) + ijk[d];
+    const long long addr = lb * 64 + lane;
+    if (contact >= 0) {
+        const double *cr = A.crec + A.contacts[contact].rec;
+        const long long S = s.nblk * 64;
+#pragma unroll
+        for (int c = 0; c < 2 * DIM; c++) out[c * nn + nid] = cr[c * S + addr];
+        return;
+    }
+    const long long gi = grid_base + addr;
+    int o = 0;
+    if (with_rec) {
+#pragma unroll
+        for (int c = 0; c < 1 + 2 * DIM; c++) out[c * nn + nid] = A.rec[c * A.nt + gi];
+        o = 1 + 2 * DIM;
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        out[(o + d) * nn + nid] = (double)outp[(G::OV + d) * A.nt + gi];
+        out[(o + DIM + d) * nn + nid] = (double)outp[(G::ODV + d) * A.nt + gi];
     }
 }
 

@@ -208,6 +208,11 @@ __global__ __launch_bounds__(256) void k_fill_i32(int *a, long long lo, long lon
     if (p < hi) a[p] = v;
 }
 
+__global__ __launch_bounds__(256) void k_fill_f64(double *a, long long n, double v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 __global__ __launch_bounds__(256) void k_unsort_act8(long long np, const int8_t *act, int8_t *dst, const int *pid) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;

@@ -28,6 +28,7 @@ SYMBOLS = (
     "mpm_gpu_get_timing", "mpm_gpu_stats", "mpm_gpu_synchronize", "mpm_gpu_destroy",
     "mpm_gpu_last_error", "mpm_gpu_footprint", "mpm_gpu_host_weights", "mpm_gpu_copy_probe",
     "mpm_gpu_set_option", "mpm_gpu_get_option", "mpm_gpu_step_batch", "mpm_gpu_load_sim", "mpm_gpu_clear_sim",
+    "mpm_gpu_download_nodes", "mpm_gpu_download_contact",
 )
 MPM_OPT_SORT_INTERVAL = 1
 MPM_OPT_FUSED_PASS = 2
@@ -37,6 +38,7 @@ MPM_OPT_PASS_GRID = 5
 MPM_OPT_GRID_WGS = 6
 MPM_OPT_GRAPH_CAPTURES = 7
 MPM_OPT_P2G_GRID = 8
+MPM_OPT_NODE_RECORD = 9
 
 
 class MpmGpuError(RuntimeError):
@@ -82,6 +84,8 @@ def load_library(path: str = LIB_PATH):
     lib.mpm_gpu_step.argtypes = [vp, C.c_double, C.c_int]
     lib.mpm_gpu_download.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, C.POINTER(C.c_int32)]
     lib.mpm_gpu_download_aux.argtypes = [vp, C.c_int, _dp, _dp]
+    lib.mpm_gpu_download_nodes.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    lib.mpm_gpu_download_contact.argtypes = [vp, C.c_int, _dp, _dp]
     lib.mpm_gpu_connectivity.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)]
     lib.mpm_gpu_set_sort_interval.argtypes = [vp, C.c_int]
     lib.mpm_gpu_set_timing.argtypes = [vp, C.c_int]
@@ -175,6 +179,7 @@ class GpuSim:
         self._keep = keep
         self._sims = sims
         self.current = [self.workloads[si].bodies[bi] for si, bi in self.body_index]
+        self.grid_N = [tuple(w.N) for w in self.workloads]  # current elements per sim
         h = C.c_void_p()
         _check(self.lib.mpm_gpu_create_batch(sims, len(self.workloads), precision, C.byref(h)))
         self.h = h
@@ -197,6 +202,7 @@ class GpuSim:
         d = _sim_desc(w, keep)
         _check(self.lib.mpm_gpu_load_sim(self.h, si, C.byref(d)))
         b0 = self.body0[si]
+        self.grid_N[si] = tuple(w.N)
         for bi, b in enumerate(w.bodies):
             self.current[b0 + bi] = b
             self.upload(b0 + bi, b)
@@ -256,6 +262,33 @@ class GpuSim:
         L = np.zeros((9, n))
         _check(self.lib.mpm_gpu_download_aux(self.h, gb, u.ctypes.data_as(_dp), L.ctypes.data_as(_dp)))
         return {"u": u.T.copy(), "L": L.T.copy()}
+
+    def _node_count(self, si):
+        nn = 1
+        for d in range(self.dim):
+            nn *= self.grid_N[si][d] + 1
+        return nn
+
+    def nodes(self, gb, record=True):
+        """Node fields of global body gb after the last step, reference node order: m (nn,),
+        mxt, f, xt, dxt (nn, dim); m, mxt, f need MPM_OPT_NODE_RECORD during that step
+        (record=False skips them)."""
+        nn = self._node_count(self.body_index[gb][0])
+        dim = self.dim
+        out = {"xt": np.zeros((dim, nn)), "dxt": np.zeros((dim, nn))}
+        if record:
+            out.update(m=np.zeros(nn), mxt=np.zeros((dim, nn)), f=np.zeros((dim, nn)))
+        ptr = lambda k: out[k].ctypes.data_as(_dp) if k in out else None
+        _check(self.lib.mpm_gpu_download_nodes(self.h, gb, ptr("m"), ptr("mxt"), ptr("f"), ptr("xt"), ptr("dxt")))
+        return {k: (v if v.ndim == 1 else v.T.copy()) for k, v in out.items()}
+
+    def contact(self, c, si=0):
+        """(normal, force) of global contact c (of sim si), each (nn, dim)."""
+        nn = self._node_count(si)
+        n = np.zeros((self.dim, nn))
+        f = np.zeros((self.dim, nn))
+        _check(self.lib.mpm_gpu_download_contact(self.h, c, n.ctypes.data_as(_dp), f.ctypes.data_as(_dp)))
+        return n.T.copy(), f.T.copy()
 
     def connectivity(self, gb):
         n = self._body(gb).n
