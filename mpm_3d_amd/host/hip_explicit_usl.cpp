@@ -118,16 +118,24 @@ void HipExplicitUSL::upload(Job *job) {
     }
 }
 
-void HipExplicitUSL::flush(Job *job) {
+// Issue the queued steps. record_last: the last of them records the node fields
+// (MPM_OPT_NODE_RECORD), which a frame with node files reads.
+void HipExplicitUSL::flush(Job *job, bool record_last) {
     if (pending > 0) {
-        long long left = pending;
+        long long left = pending - (record_last ? 1 : 0);
         while (left > 0) {
             const int k = (int)std::min<long long>(left, 1 << 30);
             check(mpm_gpu_step(handle, job->dt, k), "mpm_gpu_step");
             left -= k;
         }
+        if (record_last) {
+            check(mpm_gpu_set_option(handle, MPM_OPT_NODE_RECORD, 1), "mpm_gpu_set_option");
+            check(mpm_gpu_step(handle, job->dt, 1), "mpm_gpu_step");
+            check(mpm_gpu_set_option(handle, MPM_OPT_NODE_RECORD, 0), "mpm_gpu_set_option");
+        }
         pending = 0;
         host_stale = true;
+        nodes_stale = true;
     }
 }
 
@@ -136,9 +144,10 @@ void HipExplicitUSL::step(Job *job) {
     pending++;
 }
 
-void HipExplicitUSL::syncHost(Job *job) {
+void HipExplicitUSL::syncHost(Job *job, bool nodes) {
     if (!handle) return;
-    flush(job);
+    flush(job, nodes);
+    if (nodes && nodes_stale) syncNodes(job);
     if (!host_stale) return;
     const int dim = job->DIM;
     for (size_t b = 0; b < job->bodies.size(); b++) {
@@ -167,6 +176,44 @@ void HipExplicitUSL::syncHost(Job *job) {
         }
     }
     host_stale = false;
+}
+
+// Nodes::m, mx_t, f, x_t, diff_x_t and u = dt x_t (moveGrid, explicit_usl.cpp:233-264) of
+// every body and ContactHuang's contact_normal / contact_force, from the recorded last step.
+void HipExplicitUSL::syncNodes(Job *job) {
+    const int dim = job->DIM;
+    for (size_t b = 0; b < job->bodies.size(); b++) {
+        Nodes &N = *job->bodies[b]->nodes;
+        const size_t nn = N.size();
+        std::vector<double> m(nn), mxt(dim * nn), f(dim * nn), xt(dim * nn), dxt(dim * nn);
+        check(mpm_gpu_download_nodes(handle, (int)b, m.data(), mxt.data(), f.data(), xt.data(), dxt.data()),
+              "mpm_gpu_download_nodes");
+        for (size_t i = 0; i < nn; i++) {
+            N.m[i] = m[i];
+            for (int d = 0; d < dim; d++) {
+                N.mx_t[3 * i + d] = mxt[d * nn + i];
+                N.f[3 * i + d] = f[d * nn + i];
+                N.x_t[3 * i + d] = xt[d * nn + i];
+                N.diff_x_t[3 * i + d] = dxt[d * nn + i];
+                N.u[3 * i + d] = job->dt * N.x_t[3 * i + d];
+            }
+        }
+    }
+    for (size_t c = 0; c < job->contacts.size(); c++) {
+        auto *h = dynamic_cast<ContactHuang *>(job->contacts[c].get());
+        if (!h) continue;
+        const size_t nn = job->bodies[h->bodyIDs[0]]->nodes->size();
+        std::vector<double> nrm(dim * nn), frc(dim * nn);
+        check(mpm_gpu_download_contact(handle, (int)c, nrm.data(), frc.data()), "mpm_gpu_download_contact");
+        h->contact_normal.assign(3 * nn, 0.0);
+        h->contact_force.assign(3 * nn, 0.0);
+        for (size_t i = 0; i < nn; i++)
+            for (int d = 0; d < dim; d++) {
+                h->contact_normal[3 * i + d] = nrm[d * nn + i];
+                h->contact_force[3 * i + d] = frc[d * nn + i];
+            }
+    }
+    nodes_stale = false;
 }
 
 // The reference's ExplicitUSL returns "err" / 0 here (explicit_usl.cpp:91-99); this solver

@@ -53,8 +53,15 @@ struct Points : MPMObject {
     void writeToFile(Job *job, const std::string &file) const;
 };
 
+// DefaultNodes state (mpm_objects.hpp:164-167): positions h * ijk and all nodes active
+// (default_nodes.cpp:29-63); the other fields are those of the last step, filled by the
+// solver when a frame needs them
 struct Nodes : MPMObject {
-    virtual void init(Job *, Body *) {}
+    std::vector<double> x, u, x_t, diff_x_t, mx_t, f;  // n x 3
+    std::vector<double> m;
+    std::vector<int> active;
+    size_t size() const { return m.size(); }
+    virtual void init(Job *job, Body *body);
 };
 
 struct Material : MPMObject {
@@ -95,6 +102,7 @@ struct Contact : MPMObject {
 struct ContactHuang : Contact {
     double mu_f = 0;
     int bodyIDs[2] = {-1, -1};
+    std::vector<double> contact_normal, contact_force;  // n x 3 per node, last step (contacts.hpp:56)
     void init(Job *job) override;
 };
 
@@ -106,7 +114,13 @@ struct Grid : MPMObject {
 struct CartesianLinear : Grid {
     double L[3] = {0, 0, 0};
     int N[3] = {0, 0, 0};
+    double hx[3] = {0, 0, 0};  // L / N (cartesian.cpp:24-67)
+    std::vector<double> v_n;   // nodal volume, summed over elements as hiddenInit does
     void init(Job *job) override;
+    long long node_count() const;
+    long long element_count() const;
+    int nodeID(const int *ijk) const;               // i + (Nx+1)(j + (Ny+1) k)
+    void elementNodes(long long e, int *ids) const;  // the 2^GRID_DIM node ids of element e
 };
 
 struct Solver : MPMObject {
@@ -114,8 +128,9 @@ struct Solver : MPMObject {
     virtual void step(Job *job) = 0;
     virtual std::string saveState(Job *job, Serializer *serializer, std::string filepath) = 0;
     virtual int loadState(Job *job, Serializer *serializer, std::string fullpath) = 0;
-    // bring the host Points arrays up to date with the solver's device state
-    virtual void syncHost(Job *) {}
+    // bring the host Points arrays up to date with the solver's device state; nodes: also
+    // the Nodes and ContactHuang arrays of the last step
+    virtual void syncHost(Job *, bool nodes = false) { (void)nodes; }
 };
 
 struct Driver : MPMObject {
@@ -132,19 +147,28 @@ struct UserDefinedGravityDriver : Driver {
     void generateGravity(Job *) override {}
 };
 
-// Frame writer: "DefaultVTK" and "MinimalVTK" write one legacy-VTK point file per body
-// at the sample rate (fp64_props[0] frames/s) into str_props[0]; "NullSerializer" writes
+// Frame writers. The base class is DefaultVTK (default_vtk.cpp): at the sample rate
+// (fp64_props[0] frames/s) it writes, per body, a legacy-VTK point file fpd.<id>.<name>.<frame>.vtk
+// and a node file fnd.<...>.vtk into str_props[0], with the fields of DefaultPoints,
+// DefaultNodes, the boundary and the contacts (their writeFrame methods). MinimalVTK
+// (minimal_vtk.cpp) writes point velocity, displacement, pressure, tau_bar, gamma_dot and
+// density and node velocity and density into the same files. "NullSerializer" writes
 // nothing. writeFrame returns 1 when a frame was written.
 struct Serializer : MPMObject {
-    double sampleRate = 60;
+    double sampleRate = 60, t_last_frame = 0;
     int sampledFrames = 0;
     std::string frameDirectory, outputDirectory, simulationName;
     virtual void init(Job *job);
     virtual int writeFrame(Job *job);
+    virtual bool writesNodes() const { return true; }  // frames need the solver's node arrays
+};
+struct MinimalVTK : Serializer {
+    int writeFrame(Job *job) override;
 };
 struct NullSerializer : Serializer {
     void init(Job *) override {}
     int writeFrame(Job *) override { return 0; }
+    bool writesNodes() const override { return false; }
 };
 
 // The solver over the C-ABI. int-properties {cpdi_spec, contact_spec[, precision[, sort_interval]]}
@@ -155,15 +179,17 @@ struct HipExplicitUSL : Solver {
     mpm_gpu_handle *handle = nullptr;
     long long pending = 0;  // steps issued since the host arrays were refreshed
     bool host_stale = false;
+    bool nodes_stale = false;  // the Nodes / contact arrays predate the last step
     ~HipExplicitUSL() override;
     void init(Job *job) override;
     void step(Job *job) override;
-    void syncHost(Job *job) override;
+    void syncHost(Job *job, bool nodes = false) override;
     std::string saveState(Job *job, Serializer *serializer, std::string filepath) override;
     int loadState(Job *job, Serializer *serializer, std::string fullpath) override;
 
    private:
-    void flush(Job *job);
+    void flush(Job *job, bool record_last = false);
+    void syncNodes(Job *job);
     void upload(Job *job);
 };
 

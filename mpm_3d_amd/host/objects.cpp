@@ -57,7 +57,7 @@ template <class T, class U> static std::pair<std::string, std::function<std::uni
 
 template <> Registry<Serializer>::Registry() {
     table.insert(reg<Serializer, Serializer>("DefaultVTK"));
-    table.insert(reg<Serializer, Serializer>("MinimalVTK"));
+    table.insert(reg<Serializer, MinimalVTK>("MinimalVTK"));
     table.insert(reg<Serializer, NullSerializer>("NullSerializer"));
 }
 template <> Registry<Driver>::Registry() {
@@ -399,7 +399,9 @@ void CartesianLinear::init(Job *job) {
     for (int d = 0; d < GRID_DIM; d++) {
         L[d] = fp64_props[d];
         N[d] = int_props[d];
+        hx[d] = L[d] / N[d];
     }
+    v_n.clear();
 }
 
 // CartesianFrictionalBox::init (cartesian_frictional_box.cpp:26-37): mu_f is fp64-properties[0]
@@ -497,58 +499,6 @@ void Driver::run(Job *job) {
     clock_gettime(CLOCK_MONOTONIC, &t1);
     std::cout << "Simulation Complete. Elapsed Time [" << (t1.tv_sec - t0.tv_sec) + (t1.tv_nsec - t0.tv_nsec) * 1e-9
               << "s]." << std::endl;
-}
-
-// ------------------------------------------------------------------ point frames
-void Serializer::init(Job *) {
-    if (fp64_props.size() < 1 || str_props.size() < 3) {
-        fprintf(stderr, "%s:%s: Need at least 4 properties defined ({sampleRate},{frameDirectory, outputDirectory, simulationName}).\n",
-                __FILE__, __func__);
-        exit(0);
-    }
-    sampleRate = fp64_props[0];
-    frameDirectory = str_props[0].empty() || str_props[0].back() == '/' ? str_props[0] : str_props[0] + "/";
-    outputDirectory = str_props[1].empty() || str_props[1].back() == '/' ? str_props[1] : str_props[1] + "/";
-    simulationName = str_props[2];
-}
-
-// Frame timing of DefaultVTK::writeFrame (default_vtk.cpp:55-60): a frame whenever
-// t - t0 reaches the next sample time. Point fields as legacy VTK POLYDATA.
-int Serializer::writeFrame(Job *job) {
-    if (!((job->t - job->t0) >= (sampledFrames / sampleRate) || sampledFrames == 0)) return 0;
-    sampledFrames += 1;
-    job->solver->syncHost(job);
-    for (auto &bp : job->bodies) {
-        if (!bp || !bp->points) continue;
-        const Points &P = *bp->points;
-        std::stringstream ss;
-        ss << frameDirectory << "fpd." << bp->id << "." << bp->name << "." << std::setw(10) << std::setfill('0')
-           << (sampledFrames - 1) << ".vtk";
-        std::ofstream f(ss.str(), std::ios::trunc);
-        if (!f.is_open()) {
-            std::cerr << "Could not open point frame: " << ss.str() << " !" << std::endl;
-            continue;
-        }
-        const size_t n = P.size();
-        auto num = [](double v) { return std::isfinite(v) ? v : 0.0; };  // non-finite values written as 0
-        f << "# vtk DataFile Version 3.0\nFrame: " << (sampledFrames - 1) << ", Time: " << job->t
-          << "\nASCII\nDATASET POLYDATA\nPOINTS " << n << " double\n";
-        for (size_t i = 0; i < n; i++) f << num(P.x[3 * i]) << " " << num(P.x[3 * i + 1]) << " " << num(P.x[3 * i + 2]) << "\n";
-        f << "VERTICES " << n << " " << 2 * n << "\n";
-        for (size_t i = 0; i < n; i++) f << "1 " << i << "\n";
-        f << "POINT_DATA " << n << "\nVECTORS velocity double\n";
-        for (size_t i = 0; i < n; i++)
-            f << num(P.x_t[3 * i]) << " " << num(P.x_t[3 * i + 1]) << " " << num(P.x_t[3 * i + 2]) << "\n";
-        f << "VECTORS displacement double\n";
-        for (size_t i = 0; i < n; i++) f << num(P.u[3 * i]) << " " << num(P.u[3 * i + 1]) << " " << num(P.u[3 * i + 2]) << "\n";
-        f << "SCALARS pressure double 1\nLOOKUP_TABLE default\n";
-        for (size_t i = 0; i < n; i++) f << num(-(P.T[9 * i] + P.T[9 * i + 4] + P.T[9 * i + 8]) / 3.0) << "\n";
-        f << "SCALARS density double 1\nLOOKUP_TABLE default\n";
-        for (size_t i = 0; i < n; i++) f << num(P.m[i] / P.v[i]) << "\n";
-        f << "SCALARS active int 1\nLOOKUP_TABLE default\n";
-        for (size_t i = 0; i < n; i++) f << P.active[i] << "\n";
-    }
-    return 1;
 }
 
 }  // namespace mpmhost

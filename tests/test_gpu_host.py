@@ -70,5 +70,7 @@ def test_driver_run_with_frames(tmp_path):
     ora.step(w.dt, steps)
     _compare(w, ora, tmp_path, steps)
     frames = sorted(os.listdir(tmp_path / "output"))
-    assert len(frames) >= 120 and frames[0].startswith("fpd.0.block.")
+    pframes = [f for f in frames if f.startswith("fpd.0.block.")]
+    nframes = [f for f in frames if f.startswith("fnd.0.block.")]
+    assert len(pframes) >= 120 and len(nframes) == len(pframes) == len(frames) // 2
     assert "Simulation Complete" in r.stdout
