@@ -58,8 +58,15 @@ typedef struct {
     double gravity[3];    /* body force per unit mass, constant over the run */
 } mpm_grid_desc;
 
-/* One body. Replaces: DefaultBody with DefaultPoints/DefaultNodes, the optional
- * IsotropicLinearElasticity {E, nu} (isolin.cpp:26-46) and the body's boundary:
+/* One body. Replaces: DefaultBody with DefaultPoints/DefaultNodes, its material
+ *   has_material 0  none
+ *                1  IsotropicLinearElasticity {E, nu} (isolin.cpp:26-46)
+ *                2  Sand_SachithLocal {E, nu, grain diameter, mu_s, grains_rho, rho_critical
+ *                   [, mu_2, I_0]} with sand = {grain diameter, mu_s, grains_rho, rho_critical,
+ *                   mu_2, I_0} (mu_2 = mu_s and I_0 = 1 when the reference is given six
+ *                   properties) and sand_density_pressure for its USE_DENSITY_FOR_PRESSURE
+ *                   flag (sand_sachithlocal.cpp:58-117)
+ * and the body's boundary:
  *   has_boundary 0  none
  *                1  CartesianBox, all components fixed on the box faces (cartesian_box.cpp:23-60)
  *                2  CartesianSmoothBox, the face-normal component fixed (cartesian_smooth_box.cpp:24-80)
@@ -71,6 +78,8 @@ typedef struct {
     double E, nu;
     int has_boundary;
     double bc_mu;
+    double sand[6];
+    int sand_density_pressure;
 } mpm_body_desc;
 
 /* ContactHuang between two bodies of the same sim, {mu_f} and {body_1, body_2}
@@ -139,6 +148,12 @@ int mpm_gpu_download(mpm_gpu_handle *h, int body, double *x, double *v, double *
  * L of the last step (9 planes; zero for points without connectivity).
  * Replaces reads of Points::u and Points::L. */
 int mpm_gpu_download_aux(mpm_gpu_handle *h, int body, double *u, double *L);
+
+/* Sand_SachithLocal's per-point gammap (accumulated plastic shear strain) and gammap_dot
+ * (its rate in the last step) of one body, original order; zeros for other materials.
+ * Either pointer may be NULL. Replaces reads of Sand_SachithLocal::gammap, gammap_dot, which
+ * its writeFrame serializes (sand_sachithlocal.cpp:121-125). */
+int mpm_gpu_download_plastic(mpm_gpu_handle *h, int body, double *gammap, double *gammap_dot);
 
 /* Node fields of one body after the last step call, in the reference's node numbering
  * (CartesianLinear: n = i + (N0+1)(j + (N1+1)k), (N0+1)(N1+1)[(N2+1)] nodes for the body's

@@ -57,11 +57,13 @@ struct BodyDev {
     long long grid_base; // first node of the body's grid fields
     long long pblk_off;  // first particle-block key of the body
     long long npblk;     // particle blocks of the body
-    int has_material;
+    int has_material;    // 0 none, 1 IsotropicLinearElasticity, 2 Sand_SachithLocal
     int has_boundary;    // 0 none, 1 box (all components), 2 smooth box (normal), 3 frictional box
-    int is_b1, pad;
+    int is_b1, sand_dp;  // sand_dp: USE_DENSITY_FOR_PRESSURE
     double twoG, lambda;
     double bc_mu;        // Coulomb coefficient of the frictional box
+    double G, K;         // shear and bulk moduli
+    double sand[6];      // grain diameter, mu_s, grains_rho, rho_critical, mu_2, I_0
 };
 
 struct ContactDev {
@@ -104,13 +106,17 @@ template <> struct Cfg<2> {
 
 template <int DIM> struct PF {  // particle field planes (FP64 form)
     static constexpr int X = 0, V = DIM, T = 2 * DIM, NTS = Cfg<DIM>::NTS;
-    static constexpr int VOL = 2 * DIM + NTS, MASS = VOL + 1, EXT = VOL + 2, N = VOL + 3;
+    static constexpr int VOL = 2 * DIM + NTS, MASS = VOL + 1, EXT = VOL + 2;
+    static constexpr int GP = VOL + 3, GPD = VOL + 4;  // Sand_SachithLocal gammap, gammap_dot
+    static constexpr int N = VOL + 5;
 };
 
 // Precision-form particle planes: 4-byte words, plane 0 the packed cell index.
 template <int DIM> struct PW {
     static constexpr int CELL = 0, Q = 1, V = 1 + DIM, T = 1 + 2 * DIM, NTS = Cfg<DIM>::NTS;
-    static constexpr int LNJ = 1 + 2 * DIM + NTS, V0 = LNJ + 1, MASS = LNJ + 2, EXT = LNJ + 3, N = LNJ + 4;
+    static constexpr int LNJ = 1 + 2 * DIM + NTS, V0 = LNJ + 1, MASS = LNJ + 2, EXT = LNJ + 3;
+    static constexpr int GP = LNJ + 4, GPD = LNJ + 5;  // Sand_SachithLocal gammap, gammap_dot
+    static constexpr int N = LNJ + 6;
 };
 
 // Packed cell index of the precision form: (i + 1) | (j + 1) << B0 | (k + 1) << (B0 + B1),
@@ -434,6 +440,61 @@ __device__ __forceinline__ void isolin_update(double twoG, double lambda, double
         const R tmp = ((D[r][c] * tg + I * lt) + WT) - TW;
         T6[k] = T6[k] + tmp * (R)dt;
     }
+}
+
+// Sand_SachithLocal::calculateStress on one active point, SPEC = UPDATE (sand_sachithlocal.cpp:
+// 144-232): the Jaumann trial stress of isolin_update, then the local mu(I) return map; m / V
+// is the density after this step's volume update. Returns the plastic shear rate nup_tau.
+template <class R>
+__device__ __forceinline__ R sand_update(const BodyDev &B, double dt, R m, R V, const R L[3][3], R T6[6]) {
+    const R d = (R)B.sand[0], MU_S = (R)B.sand[1], GRAINS_RHO = (R)B.sand[2], RHO_CRITICAL = (R)B.sand[3];
+    const R MU_2 = (R)B.sand[4], I_0 = (R)B.sand[5];
+    const R G = (R)B.G, dtr = (R)dt;
+    R s[6] = {T6[0], T6[1], T6[2], T6[3], T6[4], T6[5]};
+    isolin_update<R>(B.twoG, B.lambda, dt, L, s);  // trial stress
+    R p_tr = -(s[0] + s[1] + s[2]) / (R)3.0;
+    R t0[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) t0[k] = s[k] + (k < 3 ? p_tr : (R)0);
+    // |t0| over the 9 components in row order: xx xy xz yx yy yz zx zy zz
+    R nn = t0[0] * t0[0];
+    nn += t0[3] * t0[3];
+    nn += t0[4] * t0[4];
+    nn += t0[3] * t0[3];
+    nn += t0[1] * t0[1];
+    nn += t0[5] * t0[5];
+    nn += t0[4] * t0[4];
+    nn += t0[5] * t0[5];
+    nn += t0[2] * t0[2];
+    const R tau_tr = sqrt(nn) / sqrt((R)2.0);
+    if (B.sand_dp) p_tr = (R)B.K * ((m / V - RHO_CRITICAL) / RHO_CRITICAL);
+    const bool density_flag = (m / V) < RHO_CRITICAL;
+    R nup;
+    if (density_flag || p_tr <= (R)0) {
+        nup = tau_tr / (G * dtr);
+#pragma unroll
+        for (int k = 0; k < 6; k++) T6[k] = (R)0;
+    } else {
+        R tau_tau, scale;
+        const R S0 = MU_S * p_tr;
+        if (tau_tr <= S0) {
+            tau_tau = tau_tr;
+            scale = (R)1.0;
+        } else {
+            const R S2 = MU_2 * p_tr;
+            const R alpha = G * I_0 * dtr * sqrt(p_tr / GRAINS_RHO) / d;
+            const R b = -(S2 + tau_tr + alpha);
+            const R c = S2 * tau_tr + S0 * alpha;
+            // negative_root(1, b, c) (sand_sachithlocal.cpp:44-53)
+            const R disc = sqrt(b * b - (R)4 * (R)1 * c);
+            tau_tau = b > (R)0 ? (-b - disc) / ((R)2 * (R)1) : ((R)2 * c) / (-b + disc);
+            scale = tau_tau / tau_tr;
+        }
+        nup = ((tau_tr - tau_tau) / G) / dtr;
+#pragma unroll
+        for (int k = 0; k < 6; k++) T6[k] = scale * t0[k] - (k < 3 ? p_tr : (R)0);
+    }
+    return nup;
 }
 
 }  // namespace mpm

@@ -57,6 +57,34 @@ struct Layout {  // sizes derived from the descriptors, shared by create and foo
     bool any_b1 = false;
 };
 
+// IsotropicLinearElasticity::init (isolin.cpp:34-38); Sand_SachithLocal::init computes the
+// same moduli and keeps its six properties (sand_sachithlocal.cpp:58-117)
+void set_material(BodyDev &B, const mpm_body_desc &bd) {
+    B.has_material = bd.has_material;
+    B.twoG = B.lambda = B.G = B.K = 0.0;
+    for (int k = 0; k < 6; k++) B.sand[k] = 0.0;
+    B.sand_dp = 0;
+    if (!bd.has_material) return;
+    const double G = bd.E / (2.0 * (1.0 + bd.nu));
+    const double K = bd.E / (3.0 * (1.0 - 2 * bd.nu));
+    B.G = G;
+    B.K = K;
+    B.twoG = 2 * G;
+    B.lambda = K - 2.0 * G / 3.0;
+    if (bd.has_material == 2) {
+        for (int k = 0; k < 6; k++) B.sand[k] = bd.sand[k];
+        B.sand_dp = bd.sand_density_pressure != 0;
+    }
+}
+
+int check_material(const mpm_body_desc &bd, int sim, int body) {
+    if (bd.has_material < 0 || bd.has_material > 2)
+        return fail(MPM_ERR_ARG, "sim %d body %d: has_material must be 0..2, got %d", sim, body, bd.has_material);
+    if (bd.has_material == 2 && !(bd.sand[0] > 0 && bd.sand[2] > 0 && bd.sand[3] > 0))
+        return fail(MPM_ERR_ARG, "sim %d body %d: sand needs grain diameter, grains_rho, rho_critical > 0", sim, body);
+    return MPM_OK;
+}
+
 int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
     if (nsim <= 0 || !sd) return fail(MPM_ERR_ARG, "need at least one simulation");
     Lo.dim = sd[0].grid.dim;
@@ -129,6 +157,7 @@ int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
             if (bd.n_points < 0) return fail(MPM_ERR_ARG, "negative point count");
             if (bd.has_boundary < 0 || bd.has_boundary > 3)
                 return fail(MPM_ERR_ARG, "sim %d body %d: has_boundary must be 0..3, got %d", i, b, bd.has_boundary);
+            if (int rc = check_material(bd, i, b)) return rc;
             BodyDev B{};
             B.sim = i;
             B.local = b;
@@ -137,17 +166,10 @@ int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
             B.grid_base = grid_base;
             B.pblk_off = pblk_off;
             B.npblk = pnb;
-            B.has_material = bd.has_material;
             B.has_boundary = bd.has_boundary;
             B.bc_mu = bd.bc_mu;
             B.is_b1 = is_b1[b];
-            if (bd.has_material) {
-                // IsotropicLinearElasticity::init (isolin.cpp:34-38)
-                const double G = bd.E / (2.0 * (1.0 + bd.nu));
-                const double K = bd.E / (3.0 * (1.0 - 2 * bd.nu));
-                B.twoG = 2 * G;
-                B.lambda = K - 2.0 * G / 3.0;
-            }
+            set_material(B, bd);
             p_off += bd.n_points;
             grid_base += nblk * 64;
             pblk_off += pnb;
@@ -878,6 +900,9 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
         HIP_TRY(put(FVOL, V));
         HIP_TRY(put(FMASS, m));
         HIP_TRY(put(FEXT, extent));
+        if (zeros.empty()) zeros.assign(n, 0.0);
+        HIP_TRY(put(FEXT + 1, zeros.data()));  // gammap, gammap_dot start at 0
+        HIP_TRY(put(FEXT + 2, zeros.data()));
         HIP_TRY(hipStreamSynchronize(h->stream));
         swap_buffers(h);
     }
@@ -918,6 +943,7 @@ int mpm_gpu_load_sim(mpm_gpu_handle *h, int sim, const mpm_sim_desc *d) {
             return fail(MPM_ERR_ARG, "sim %d body %d: at most %lld points", sim, b, (long long)h->lo.bodies[s.body0 + b].n);
         if (d->bodies[b].has_boundary < 0 || d->bodies[b].has_boundary > 3)
             return fail(MPM_ERR_ARG, "sim %d body %d: has_boundary must be 0..3, got %d", sim, b, d->bodies[b].has_boundary);
+        if (int rc = check_material(d->bodies[b], sim, b)) return rc;
     }
     for (int c = 0; c < d->ncontact; c++) {
         const mpm_contact_desc &cd = d->contacts[c];
@@ -950,13 +976,9 @@ int mpm_gpu_load_sim(mpm_gpu_handle *h, int sim, const mpm_sim_desc *d) {
         B.is_b1 = 0;
         if (b < d->nbody) {
             const mpm_body_desc &bd = d->bodies[b];
-            B.has_material = bd.has_material;
             B.has_boundary = bd.has_boundary;
             B.bc_mu = bd.bc_mu;
-            const double G = bd.E / (2.0 * (1.0 + bd.nu));
-            const double K = bd.E / (3.0 * (1.0 - 2 * bd.nu));
-            B.twoG = bd.has_material ? 2 * G : 0.0;
-            B.lambda = bd.has_material ? K - 2.0 * G / 3.0 : 0.0;
+            set_material(B, bd);
             h->count[gb] = bd.n_points;
         } else {
             h->count[gb] = 0;
@@ -1227,6 +1249,45 @@ int mpm_gpu_download_aux(mpm_gpu_handle *h, int body, double *u, double *L) {
                                    hipMemcpyDeviceToHost, h->stream));
         HIP_TRY(hipStreamSynchronize(h->stream));
     }
+    return MPM_OK;
+}
+
+int mpm_gpu_download_plastic(mpm_gpu_handle *h, int body, double *gammap, double *gammap_dot) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    if (body < 0 || body >= (int)h->lo.bodies.size()) return fail(MPM_ERR_ARG, "body %d out of range", body);
+    const BodyDev &B = h->lo.bodies[body];
+    const int dim = h->lo.dim;
+    const long long n = h->count[body], np = h->lo.np;
+    if (n == 0) return MPM_OK;
+    const bool f32 = h->precision == MPM_PREC_FP32;
+    const int gpf = f32 ? (dim == 3 ? PW<3>::GP : PW<2>::GP) : (dim == 3 ? PF<3>::GP : PF<2>::GP);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    int rc = ensure_stage(h, np * 2);
+    if (rc) return rc;
+    const int gp = (int)ceil_div(np, 256);
+    if (f32) {
+        uint32_t *st = reinterpret_cast<uint32_t *>(h->stage);
+        hipLaunchKernelGGL(k_unsort<uint32_t>, dim3(gp), dim3(256), 0, h->stream, np, 2, h->Pw + (long long)gpf * np, np,
+                           st, h->pid);
+        HIP_TRY(hipGetLastError());
+        std::vector<float> g(2 * n);
+        for (int f = 0; f < 2; f++)
+            HIP_TRY(hipMemcpyAsync(g.data() + (size_t)f * n, st + (long long)f * np + B.p_off, sizeof(float) * n,
+                                   hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        for (long long i = 0; i < n; i++) {
+            if (gammap) gammap[i] = g[i];
+            if (gammap_dot) gammap_dot[i] = g[n + i];
+        }
+        return MPM_OK;
+    }
+    hipLaunchKernelGGL(k_unsort<double>, dim3(gp), dim3(256), 0, h->stream, np, 2, h->P + (long long)gpf * np, np,
+                       h->stage, h->pid);
+    HIP_TRY(hipGetLastError());
+    if (gammap) HIP_TRY(hipMemcpyAsync(gammap, h->stage + B.p_off, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    if (gammap_dot)
+        HIP_TRY(hipMemcpyAsync(gammap_dot, h->stage + np + B.p_off, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
     return MPM_OK;
 }
 

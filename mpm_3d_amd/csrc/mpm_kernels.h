@@ -353,9 +353,11 @@ void k_particles(KArgs A, int want_L) {
         // registers than it saves in latency.)
         St qn;
         bool an = false;
+        // the volume is needed by P2G and by the sand's density (G2P of a sand body)
+        const bool need_vol = DO_P2G || B.has_material == 2;
         if (PREFETCH && tid < count) {
             an = A.act[start + tid] != 0;
-            if (an) PT::template load<DIM>(A, start + tid, qn, DO_P2G);
+            if (an) PT::template load<DIM>(A, start + tid, qn, need_vol);
         }
         const int lane = tid & 63;
 #pragma unroll 1
@@ -385,11 +387,11 @@ void k_particles(KArgs A, int want_L) {
                 an = false;
                 if (kn < count) {
                     an = A.act[start + kn] != 0;
-                    if (an) PT::template load<DIM>(A, start + kn, qn, DO_P2G);
+                    if (an) PT::template load<DIM>(A, start + kn, qn, need_vol);
                 }
             } else {
                 a = k < count && A.act[start + k] != 0;
-                if (a) PT::template load<DIM>(A, start + k, q, DO_P2G);
+                if (a) PT::template load<DIM>(A, start + k, q, need_vol);
             }
             if (ORDER && PT::FP32) {
                 // FP32 state: load in slot order, then move the loaded points across the
@@ -489,6 +491,7 @@ void k_particles(KArgs A, int want_L) {
                 }
                 PT::template advance<DIM>(B, s, dt, q, up, dvp, Lp);
                 PT::template store<DIM>(A, p, q);
+                if (B.has_material == 2) PT::template store_plastic<DIM>(A, p, q.nup, dt);
                 if (want_L && A.Lout) {
                     double L9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll

@@ -27,6 +27,7 @@ struct Prec64 {
     static constexpr bool FP32 = false;
     template <int DIM> struct State {
         double x[DIM], v[DIM], T6[6], V, m, ext;
+        double nup;  // Sand_SachithLocal plastic shear rate of the last update
     };
 
     template <int DIM>
@@ -91,8 +92,17 @@ struct Prec64 {
             for (int r = 0; r < DIM; r++)
 #pragma unroll
                 for (int c = 0; c < DIM; c++) L3[r][c] = Lp[r][c];
-            isolin_update<double>(B.twoG, B.lambda, dt, L3, q.T6);
+            if (B.has_material == 2) q.nup = sand_update<double>(B, dt, q.m, q.V, L3, q.T6);
+            else isolin_update<double>(B.twoG, B.lambda, dt, L3, q.T6);
         }
+    }
+
+    // Sand_SachithLocal: gammap += nup dt, gammap_dot = nup (sand_sachithlocal.cpp:226-229)
+    template <int DIM>
+    __device__ __forceinline__ static void store_plastic(const KArgs &A, long long p, double nup, double dt) {
+        using F = PF<DIM>;
+        A.P[F::GP * A.np + p] = A.P[F::GP * A.np + p] + nup * dt;
+        A.P[F::GPD * A.np + p] = nup;
     }
 
     // sort key inputs: dual cell (shifted to >= 0) and octant inside it; false if the
@@ -161,7 +171,8 @@ struct Prec32 {
         int c[DIM];       // cell
         uint32_t q[DIM];  // offset inside the cell, 32-bit fixed point (x = h (c + q 2^-32))
         float r[DIM];     // q 2^-32 rounded to FP32, for the shape functions
-        float v[DIM], T6[6], lnJ, V, m, ext;  // V = V0 exp(lnJ), loaded for P2G only
+        float v[DIM], T6[6], lnJ, V, m, ext;  // V = V0 exp(lnJ), loaded for P2G and sand only
+        float nup;  // Sand_SachithLocal plastic shear rate of the last update
     };
 
     template <int DIM>
@@ -379,8 +390,18 @@ struct Prec32 {
             for (int r = 0; r < DIM; r++)
 #pragma unroll
                 for (int c = 0; c < DIM; c++) L3[r][c] = Lp[r][c];
-            isolin_update<float>(B.twoG, B.lambda, dt, L3, q.T6);
+            if (B.has_material == 2) q.nup = sand_update<float>(B, dt, q.m, q.V, L3, q.T6);
+            else isolin_update<float>(B.twoG, B.lambda, dt, L3, q.T6);
         }
+    }
+
+    // Sand_SachithLocal: gammap += nup dt, gammap_dot = nup (sand_sachithlocal.cpp:226-229)
+    template <int DIM>
+    __device__ __forceinline__ static void store_plastic(const KArgs &A, long long p, float nup, double dt) {
+        using F = PW<DIM>;
+        float *Pf = reinterpret_cast<float *>(A.Pw);
+        Pf[F::GP * A.np + p] = Pf[F::GP * A.np + p] + nup * (float)dt;
+        Pf[F::GPD * A.np + p] = nup;
     }
 
     template <int DIM>

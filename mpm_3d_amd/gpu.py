@@ -28,7 +28,7 @@ SYMBOLS = (
     "mpm_gpu_get_timing", "mpm_gpu_stats", "mpm_gpu_synchronize", "mpm_gpu_destroy",
     "mpm_gpu_last_error", "mpm_gpu_footprint", "mpm_gpu_host_weights", "mpm_gpu_copy_probe",
     "mpm_gpu_set_option", "mpm_gpu_get_option", "mpm_gpu_step_batch", "mpm_gpu_load_sim", "mpm_gpu_clear_sim",
-    "mpm_gpu_download_nodes", "mpm_gpu_download_contact",
+    "mpm_gpu_download_nodes", "mpm_gpu_download_contact", "mpm_gpu_download_plastic",
 )
 MPM_OPT_SORT_INTERVAL = 1
 MPM_OPT_FUSED_PASS = 2
@@ -52,7 +52,8 @@ class GridDesc(C.Structure):
 
 class BodyDesc(C.Structure):
     _fields_ = [("n_points", C.c_int64), ("has_material", C.c_int), ("E", C.c_double), ("nu", C.c_double),
-                ("has_boundary", C.c_int), ("bc_mu", C.c_double)]
+                ("has_boundary", C.c_int), ("bc_mu", C.c_double), ("sand", C.c_double * 6),
+                ("sand_density_pressure", C.c_int)]
 
 
 class ContactDesc(C.Structure):
@@ -86,6 +87,7 @@ def load_library(path: str = LIB_PATH):
     lib.mpm_gpu_download_aux.argtypes = [vp, C.c_int, _dp, _dp]
     lib.mpm_gpu_download_nodes.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, _dp]
     lib.mpm_gpu_download_contact.argtypes = [vp, C.c_int, _dp, _dp]
+    lib.mpm_gpu_download_plastic.argtypes = [vp, C.c_int, _dp, _dp]
     lib.mpm_gpu_connectivity.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)]
     lib.mpm_gpu_set_sort_interval.argtypes = [vp, C.c_int]
     lib.mpm_gpu_set_timing.argtypes = [vp, C.c_int]
@@ -145,6 +147,12 @@ def _sim_desc(w, keep):
         bds[bi].nu = float(b.nu or 0.0)
         bds[bi].has_boundary = int(b.boundary)
         bds[bi].bc_mu = float(getattr(b, "bc_mu", 0.0))
+        sand = getattr(b, "sand", None)
+        if sand is not None:
+            bds[bi].has_material = 2
+            for k in range(6):
+                bds[bi].sand[k] = float(sand[k])
+            bds[bi].sand_density_pressure = int(bool(getattr(b, "sand_dp", False)))
     cds = (ContactDesc * max(1, len(w.contacts)))()
     for ci, c in enumerate(w.contacts):
         cds[ci].body1, cds[ci].body2, cds[ci].mu = c.body1, c.body2, c.mu
@@ -262,6 +270,13 @@ class GpuSim:
         L = np.zeros((9, n))
         _check(self.lib.mpm_gpu_download_aux(self.h, gb, u.ctypes.data_as(_dp), L.ctypes.data_as(_dp)))
         return {"u": u.T.copy(), "L": L.T.copy()}
+
+    def plastic(self, gb):
+        """Sand_SachithLocal gammap, gammap_dot of global body gb (original order)."""
+        n = self._body(gb).n
+        gp, gpd = np.zeros(n), np.zeros(n)
+        _check(self.lib.mpm_gpu_download_plastic(self.h, gb, gp.ctypes.data_as(_dp), gpd.ctypes.data_as(_dp)))
+        return gp, gpd
 
     def _node_count(self, si):
         nn = 1

@@ -342,8 +342,14 @@ int Serializer::writeFrame(Job *job) {
             fr.vectors(Nd.mx_t, "momentum");
             fr.vectors(Nd.f, "force");
             fr.scalars(as_double(Nd.active), "active");
-            // IsotropicLinearElasticity and CartesianFrictionalBox write nothing; the two
-            // other boxes their mask (cartesian_box.cpp:90-93, cartesian_smooth_box.cpp:86-89)
+            // Sand_SachithLocal::writeFrame (sand_sachithlocal.cpp:121-125); isolin writes nothing
+            if (B.activeMaterial != 0)
+                if (auto *sand = dynamic_cast<Sand_SachithLocal *>(B.material.get())) {
+                    fr.scalars(sand->gammap, "gammap");
+                    fr.scalars(sand->gammap_dot, "gammap_dot");
+                }
+            // CartesianFrictionalBox writes nothing; the two other boxes their mask
+            // (cartesian_box.cpp:90-93, cartesian_smooth_box.cpp:86-89)
             if (B.activeBoundary != 0) {
                 if (dynamic_cast<CartesianBox *>(B.boundary.get())) fr.vectors(box_mask(job, Nd, true), "bc_nodal_mask");
                 else if (dynamic_cast<CartesianSmoothBox *>(B.boundary.get()))

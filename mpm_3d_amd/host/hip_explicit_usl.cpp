@@ -67,11 +67,22 @@ void HipExplicitUSL::upload(Job *job) {
         mpm_body_desc b{};
         b.n_points = (int64_t)bp->points->size();
         if (bp->activeMaterial) {
-            auto *mat = dynamic_cast<IsotropicLinearElasticity *>(bp->material.get());
-            if (!mat) refuse(__func__, "HipExplicitUSL supports the IsotropicLinearElasticity material only.");
-            b.has_material = 1;
-            b.E = mat->E;
-            b.nu = mat->nu;
+            if (auto *mat = dynamic_cast<IsotropicLinearElasticity *>(bp->material.get())) {
+                b.has_material = 1;
+                b.E = mat->E;
+                b.nu = mat->nu;
+            } else if (auto *sand = dynamic_cast<Sand_SachithLocal *>(bp->material.get())) {
+                b.has_material = 2;
+                b.E = sand->E;
+                b.nu = sand->nu;
+                const double sp[6] = {sand->grains_d, sand->MU_S, sand->GRAINS_RHO, sand->RHO_CRITICAL, sand->MU_2,
+                                      sand->I_0};
+                for (int k = 0; k < 6; k++) b.sand[k] = sp[k];
+                b.sand_density_pressure = sand->use_density_to_determine_pressure ? 1 : 0;
+            } else {
+                refuse(__func__, "HipExplicitUSL supports the IsotropicLinearElasticity and Sand_SachithLocal "
+                                 "materials only.");
+            }
         }
         if (bp->activeBoundary) {
             Boundary *bc = bp->boundary.get();
@@ -173,6 +184,12 @@ void HipExplicitUSL::syncHost(Job *job, bool nodes) {
             for (int k = 0; k < 9; k++) P.L[9 * i + k] = L[k * n + i];
             P.v[i] = V[i];
             P.active[i] = act[i];
+        }
+        if (auto *sand = dynamic_cast<Sand_SachithLocal *>(job->bodies[b]->material.get())) {
+            sand->gammap.assign(n, 0.0);
+            sand->gammap_dot.assign(n, 0.0);
+            check(mpm_gpu_download_plastic(handle, (int)b, sand->gammap.data(), sand->gammap_dot.data()),
+                  "mpm_gpu_download_plastic");
         }
     }
     host_stale = false;

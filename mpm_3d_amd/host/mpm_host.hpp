@@ -73,6 +73,17 @@ struct IsotropicLinearElasticity : Material {
     void init(Job *job) override;
 };
 
+// Sand_SachithLocal (materials.hpp:85-120): {E, nu, grain diameter, mu_s, grains_rho,
+// rho_critical[, mu_2, I_0]}, str-properties USE_DENSITY_FOR_PRESSURE; gammap and
+// gammap_dot are filled by the solver when a frame needs them
+struct Sand_SachithLocal : Material {
+    double E = 0, nu = 0, G = 0, K = 0, lambda = 0, grains_d = 0;
+    double MU_S = 0, GRAINS_RHO = 0, RHO_CRITICAL = 0, MU_2 = 0, DELTA_MU = 0, I_0 = 1;
+    bool use_density_to_determine_pressure = false;
+    std::vector<double> gammap, gammap_dot;
+    void init(Job *job) override;
+};
+
 struct Boundary : MPMObject {
     virtual void init(Job *, Body *) {}
 };

@@ -83,6 +83,7 @@ template <> Registry<Nodes>::Registry() {
 template <> Registry<Contact>::Registry() { table.insert(reg<Contact, ContactHuang>("ContactHuang")); }
 template <> Registry<Material>::Registry() {
     table.insert(reg<Material, IsotropicLinearElasticity>("IsotropicLinearElasticity"));
+    table.insert(reg<Material, Sand_SachithLocal>("Sand_SachithLocal"));
 }
 template <> Registry<Boundary>::Registry() {
     table.insert(reg<Boundary, CartesianBox>("CartesianBox"));
@@ -388,6 +389,46 @@ void IsotropicLinearElasticity::init(Job *) {
     K = E / (3.0 * (1.0 - 2 * nu));
     lambda = K - 2.0 * G / 3.0;
     printf("Material properties (E = %g, nu = %g, G = %g, K = %g).\n", E, nu, G, K);
+}
+
+// Sand_SachithLocal::init (sand_sachithlocal.cpp:58-117), including its messages and its
+// exit(EXIT_FAILURE) on fewer than six properties
+void Sand_SachithLocal::init(Job *) {
+    if (fp64_props.size() < 6) {
+        fprintf(stderr,
+                "%s:%s: Need at least 6 properties defined (E, nu, grain diameter, mu_s, grains_rho, rho_critical).\n",
+                __FILE__, __func__);
+        exit(EXIT_FAILURE);
+    }
+    E = fp64_props[0];
+    nu = fp64_props[1];
+    grains_d = fp64_props[2];
+    G = E / (2.0 * (1.0 + nu));
+    K = E / (3.0 * (1.0 - 2 * nu));
+    lambda = K - 2.0 * G / 3.0;
+    MU_S = fp64_props[3];
+    GRAINS_RHO = fp64_props[4];
+    RHO_CRITICAL = fp64_props[5];
+    if (fp64_props.size() < 8) {
+        MU_2 = MU_S;
+        DELTA_MU = MU_2 - MU_S;
+        I_0 = 1;
+        printf("Material properties (E = %g, nu = %g, G = %g, K = %g, grain diameter = %g, mu_s = %g, grains_rho = %g, "
+               "rho_critical = %g).\n",
+               E, nu, G, K, grains_d, MU_S, GRAINS_RHO, RHO_CRITICAL);
+    } else {
+        MU_2 = fp64_props[6];
+        DELTA_MU = MU_2 - MU_S;
+        I_0 = fp64_props[7];
+        printf("Material properties (E = %g, nu = %g, G = %g, K = %g, grain diameter = %g, mu_s = %g, grains_rho = %g, "
+               "rho_critical = %g, mu_2 = %g, i_0 = %g).\n",
+               E, nu, G, K, grains_d, MU_S, GRAINS_RHO, RHO_CRITICAL, MU_2, I_0);
+    }
+    for (const std::string &sp : str_props)
+        if (sp == "USE_DENSITY_FOR_PRESSURE") {
+            use_density_to_determine_pressure = true;
+            std::cout << "Using density to determine pressure." << std::endl;
+        }
 }
 
 void CartesianLinear::init(Job *job) {

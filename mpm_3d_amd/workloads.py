@@ -36,6 +36,10 @@ class Body:
     nu: Optional[float] = 0.3
     boundary: int = 1       # 0 none, 1 CartesianBox, 2 CartesianSmoothBox, 3 CartesianFrictionalBox
     bc_mu: float = 0.0      # CartesianFrictionalBox mu_f
+    # Sand_SachithLocal instead of IsotropicLinearElasticity (E, nu as above):
+    # (grain diameter, mu_s, grains_rho, rho_critical, mu_2, I_0) and USE_DENSITY_FOR_PRESSURE
+    sand: Optional[tuple] = None
+    sand_dp: bool = False
 
     @property
     def n(self) -> int:
@@ -162,6 +166,24 @@ def sliding_block(dim=2, boundary=3, mu=0.3, wall_dim=None, upper=False, speed=1
     face = f"{'upper' if upper else 'lower'}{wall_dim}"
     return Workload(f"sliding_block_{dim}d_{face}_bc{boundary}_mu{mu}", dim, L, N, [body], [], 1e-4, steps,
                     tuple(grav))
+
+
+def sand_props(mu_s=0.577, grains_d=0.01, grains_rho=1650.0, rho_critical=1000.0, mu_2=None, I_0=1.0):
+    """Sand_SachithLocal's material properties after E, nu, in the order the C-ABI takes them.
+    With six properties the reference sets mu_2 = mu_s and I_0 = 1 (sand_sachithlocal.cpp:66-79)."""
+    return (grains_d, mu_s, grains_rho, rho_critical, mu_s if mu_2 is None else mu_2, I_0)
+
+
+def column_collapse_example(dim, steps=100):
+    """The shipped columncollapse2D / columncollapse3D examples: their point files with the
+    configuration's grid (1 x 0.5 m at 40 x 20; 1 x 1 x 0.5 m at 20 x 20 x 10), dt = 1e-4,
+    Sand_SachithLocal {1e7, 0.3, 0.01, 0.577, 1650, 1000} and CartesianBox walls."""
+    name = f"columncollapse{dim}D"
+    m, v, x, xt, a = read_points(os.path.join(example_dir(), name, f"test{dim}D.points"), dim)
+    body = Body("block", m, v, x, xt, a, 1e7, 0.3, 1, 0.0, sand_props())
+    L, N = ((1.0, 0.5), (40, 20)) if dim == 2 else ((1.0, 1.0, 0.5), (20, 20, 10))
+    g = (0.0, -9.81) if dim == 2 else (0.0, 0.0, -9.81)
+    return Workload(name, dim, L, N, [body], [], 1e-4, steps, g)
 
 
 def column_3d(wide=True, steps=1000, seed=0xC0FFEE + 2, grid=(256, 256, 128), domain=(2.0, 2.0, 1.0),

@@ -81,8 +81,13 @@ typedef struct {
     double *m, *v, *v0, *extent;  /* n */
     int32_t *active;
 
-    int has_material;
+    int has_material;  /* 1 IsotropicLinearElasticity, 2 Sand_SachithLocal */
     double E, nu, Gmod, K, lambda;
+    /* Sand_SachithLocal {grain diameter, mu_s, grains_rho, rho_critical, mu_2, I_0} and
+       USE_DENSITY_FOR_PRESSURE (sand_sachithlocal.cpp:58-117); plastic strain and its rate */
+    double sand[6];
+    int sand_dp;
+    double *gammap, *gammap_dot;
     int has_boundary;  /* 1 CartesianBox, 2 CartesianSmoothBox, 3 CartesianFrictionalBox */
     double bc_mu;      /* CartesianFrictionalBox mu_f */
     double *bc_mask;   /* 3 per node (cartesian_box.cpp:53-59, cartesian_smooth_box.cpp:40-50,
@@ -775,11 +780,74 @@ static void isolin_point(double G, double lambda, double dt, const double *L, do
     for (int k = 0; k < 9; k++) T[k] = T[k] + tmp[k] * dt;
 }
 
+/* Sand_SachithLocal::negative_root (sand_sachithlocal.cpp:44-53): the smaller root of
+ * a x^2 + b x + c without cancellation */
+static double negative_root(double a, double b, double c) {
+    double x;
+    if (b > 0) x = (-b - sqrt(b * b - 4 * a * c)) / (2 * a);
+    else x = (2 * c) / (-b + sqrt(b * b - 4 * a * c));
+    return x;
+}
+
+/* Sand_SachithLocal::calculateStress on one point, SPEC = UPDATE (sand_sachithlocal.cpp:
+ * 144-232): the Jaumann trial stress of isolin_point, then the local mu(I) return map.
+ * m / v is the point's density after this step's volume update. Returns nup_tau. */
+static double sand_point(double G, double K, double lambda, const double *sp, int use_dp, double dt, double m,
+                         double v, const double *L, double *T) {
+    const double c = 0;
+    const double d = sp[0], MU_S = sp[1], GRAINS_RHO = sp[2], RHO_CRITICAL = sp[3], MU_2 = sp[4], I_0 = sp[5];
+    double s_tr[9], t0_tr[9];
+    memcpy(s_tr, T, sizeof(s_tr));
+    isolin_point(G, lambda, dt, L, s_tr);  /* s_tr = T + dt (2G D + lambda trD I + W T - T W) */
+    double p_tr = -(s_tr[0] + s_tr[4] + s_tr[8]) / 3.0;
+    double nn = 0;
+    for (int k = 0; k < 9; k++) {
+        const double Ik = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+        t0_tr[k] = s_tr[k] + p_tr * Ik;
+    }
+    for (int k = 0; k < 9; k++) nn += t0_tr[k] * t0_tr[k];
+    const double tau_tr = sqrt(nn) / sqrt(2.0);
+    if (use_dp) p_tr = K * ((m / v - RHO_CRITICAL) / RHO_CRITICAL);
+    const int density_flag = (m / v) < RHO_CRITICAL;
+    double nup_tau;
+    if (density_flag || p_tr <= c) {
+        nup_tau = tau_tr / (G * dt);
+        for (int k = 0; k < 9; k++) T[k] = 0;
+    } else {
+        double tau_tau, scale_factor;
+        const double S0 = MU_S * p_tr;
+        if (tau_tr <= S0) {
+            tau_tau = tau_tr;
+            scale_factor = 1.0;
+        } else {
+            const double S2 = MU_2 * p_tr;
+            const double alpha = G * I_0 * dt * sqrt(p_tr / GRAINS_RHO) / d;
+            const double Bq = -(S2 + tau_tr + alpha);
+            const double H = S2 * tau_tr + S0 * alpha;
+            tau_tau = negative_root(1.0, Bq, H);
+            scale_factor = (tau_tau / tau_tr);
+        }
+        nup_tau = ((tau_tr - tau_tau) / G) / dt;
+        for (int k = 0; k < 9; k++) {
+            const double Ik = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+            T[k] = scale_factor * t0_tr[k] - p_tr * Ik;
+        }
+    }
+    return nup_tau;
+}
+
 static void update_stress(ora_sim *s, ora_body *B) {
     if (!B->has_material) return;
     for (int64_t i = 0; i < B->n; i++) {
         if (B->active[i] == 0) continue;
-        isolin_point(B->Gmod, B->lambda, s->dt, &B->L[9 * i], &B->T[9 * i]);
+        if (B->has_material == 2) {
+            const double nup = sand_point(B->Gmod, B->K, B->lambda, B->sand, B->sand_dp, s->dt, B->m[i], B->v[i],
+                                          &B->L[9 * i], &B->T[9 * i]);
+            B->gammap[i] += nup * s->dt;
+            B->gammap_dot[i] = nup;
+        } else {
+            isolin_point(B->Gmod, B->lambda, s->dt, &B->L[9 * i], &B->T[9 * i]);
+        }
     }
 }
 
@@ -866,6 +934,8 @@ int ora_add_body(ora_sim *s, int64_t n, const double *m, const double *v, const 
     B->v0 = (double *)calloc(n, sizeof(double));
     B->extent = (double *)calloc(n, sizeof(double));
     B->active = (int32_t *)calloc(n, sizeof(int32_t));
+    B->gammap = (double *)calloc(n, sizeof(double));
+    B->gammap_dot = (double *)calloc(n, sizeof(double));
     memcpy(B->m, m, sizeof(double) * n);
     memcpy(B->v, v, sizeof(double) * n);
     memcpy(B->x, x, sizeof(double) * 3 * n);
@@ -1083,6 +1153,32 @@ int ora_basis_gradient(const ora_sim *s, const double *x, int32_t *nid, double *
     return basis_gradient(s, x, nid, grad3);
 }
 
+int ora_set_sand(ora_sim *s, int body, const double *props6, int use_density_pressure) {
+    if (body < 0 || body >= s->nbody) return -1;
+    ora_body *B = &s->bodies[body];
+    if (!B->has_material) return -1;
+    B->has_material = 2;
+    memcpy(B->sand, props6, sizeof(B->sand));
+    B->sand_dp = use_density_pressure;
+    return 0;
+}
+
+void ora_get_plastic(const ora_sim *s, int body, double *gammap, double *gammap_dot) {
+    const ora_body *B = &s->bodies[body];
+    for (int64_t i = 0; i < B->n; i++) {
+        if (gammap) gammap[i] = B->gammap[i];
+        if (gammap_dot) gammap_dot[i] = B->gammap_dot[i];
+    }
+}
+
+double ora_sand_update(double E, double nu, const double *props6, int use_density_pressure, double dt, double m,
+                       double v, const double *L9, double *T9) {
+    double G = E / (2.0 * (1.0 + nu));
+    double K = E / (3.0 * (1.0 - 2 * nu));
+    double lambda = K - 2.0 * G / 3.0;
+    return sand_point(G, K, lambda, props6, use_density_pressure, dt, m, v, L9, T9);
+}
+
 void ora_isolin_update(double E, double nu, double dt, const double *L9, double *T9) {
     double G = E / (2.0 * (1.0 + nu));
     double K = E / (3.0 * (1.0 - 2 * nu));
@@ -1096,7 +1192,7 @@ void ora_destroy(ora_sim *s) {
         ora_body *B = &s->bodies[b];
         free(B->x); free(B->u); free(B->xt); free(B->mxt); free(B->b);
         free(B->L); free(B->T); free(B->m); free(B->v); free(B->v0); free(B->extent);
-        free(B->active); free(B->bc_mask); free(B->bc_force);
+        free(B->active); free(B->bc_mask); free(B->bc_force); free(B->gammap); free(B->gammap_dot);
         free(B->nm); free(B->nx); free(B->nu_); free(B->nxt); free(B->ndxt); free(B->nmxt); free(B->nf);
         coo_free(&B->S);
         coo_free(&B->G);

@@ -90,6 +90,15 @@ int ora_basis_gradient(const ora_sim *s, const double *x, int32_t *nid, double *
 /* IsotropicLinearElasticity::calculateStress on a single tensor (isolin.cpp:50-78) */
 void ora_isolin_update(double E, double nu, double dt, const double *L9, double *T9);
 
+/* Sand_SachithLocal (sand_sachithlocal.cpp): make body `body` (added with a material, before
+ * ora_init) a sand with props6 = {grain diameter, mu_s, grains_rho, rho_critical, mu_2, I_0}
+ * and USE_DENSITY_FOR_PRESSURE; its plastic strain gammap and rate gammap_dot (per point);
+ * one point's update (returns nup_tau; T9 updated in place). */
+int ora_set_sand(ora_sim *s, int body, const double *props6, int use_density_pressure);
+void ora_get_plastic(const ora_sim *s, int body, double *gammap, double *gammap_dot);
+double ora_sand_update(double E, double nu, const double *props6, int use_density_pressure, double dt, double m,
+                       double v, const double *L9, double *T9);
+
 void ora_destroy(ora_sim *s);
 
 #ifdef __cplusplus

@@ -67,6 +67,11 @@ def _lib(native: bool = False):
     lib.ora_basis_gradient.restype = C.c_int
     lib.ora_basis_gradient.argtypes = [C.c_void_p, _dp, _ip, _dp]
     lib.ora_isolin_update.argtypes = [C.c_double, C.c_double, C.c_double, _dp, _dp]
+    lib.ora_set_sand.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int]
+    lib.ora_get_plastic.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    lib.ora_sand_update.argtypes = [C.c_double, C.c_double, _dp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    _dp, _dp]
+    lib.ora_sand_update.restype = C.c_double
     lib.ora_destroy.argtypes = [C.c_void_p]
     _LIBS[name] = lib
     return lib
@@ -115,6 +120,19 @@ class OracleSim:
                                   float(E or 0.0), float(nu or 0.0), int(boundary), float(bc_mu))
         self.sizes.append(n)
         return b
+
+    def set_sand(self, body, props6, use_density_pressure=False):
+        """Sand_SachithLocal for body (added with E, nu; before init): props6 = (grain
+        diameter, mu_s, grains_rho, rho_critical, mu_2, I_0)"""
+        p = np.ascontiguousarray(props6, dtype=np.float64)
+        if self.lib.ora_set_sand(self.h, body, _d(p), int(bool(use_density_pressure))) != 0:
+            raise ValueError("oracle rejected the sand material")
+
+    def plastic(self, body):
+        n = self.sizes[body]
+        gp, gpd = np.zeros(n), np.zeros(n)
+        self.lib.ora_get_plastic(self.h, body, _d(gp), _d(gpd))
+        return gp, gpd
 
     def add_contact(self, b1, b2, mu):
         return self.lib.ora_add_contact(self.h, b1, b2, mu)
@@ -242,3 +260,13 @@ def isolin_update(E, nu, dt, L9, T9):
     T = np.ascontiguousarray(np.asarray(T9, np.float64).reshape(9).copy())
     lib.ora_isolin_update(E, nu, dt, _d(L), _d(T))
     return T
+
+
+def sand_update(E, nu, props6, dt, m, v, L9, T9, use_density_pressure=False):
+    """Sand_SachithLocal::calculateStress on one point; returns (T9 after, nup_tau)."""
+    lib = _lib()
+    p = np.ascontiguousarray(props6, dtype=np.float64)
+    L = np.ascontiguousarray(L9, dtype=np.float64).reshape(9)
+    T = np.ascontiguousarray(T9, dtype=np.float64).reshape(9).copy()
+    nup = lib.ora_sand_update(E, nu, _d(p), int(bool(use_density_pressure)), dt, m, v, _d(L), _d(T))
+    return T, nup

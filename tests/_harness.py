@@ -29,8 +29,10 @@ VOLTOL = 1e-12
 def oracle_of(w, native=False):
     from oracle.pyoracle import OracleSim
     s = OracleSim(w.dim, w.L, w.N, w.cpdi_spec, w.contact_spec, native=native)
-    for b in w.bodies:
+    for bi, b in enumerate(w.bodies):
         s.add_body(b.m, b.v, b.x, b.xt, b.active, E=b.E, nu=b.nu, boundary=b.boundary, bc_mu=b.bc_mu)
+        if getattr(b, "sand", None) is not None:
+            s.set_sand(bi, b.sand, b.sand_dp)
     for c in w.contacts:
         s.add_contact(c.body1, c.body2, c.mu)
     s.init(g=w.gravity)

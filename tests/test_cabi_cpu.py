@@ -83,7 +83,7 @@ def test_body_descriptor_layout_and_boundary_check():
     body's fields at the wrong offset), and has_boundary outside 0..3 is rejected with a
     message naming the body."""
     lib = G.load_library()
-    assert C.sizeof(G.BodyDesc) == 48
+    assert C.sizeof(G.BodyDesc) == 104
     w = W.two_block_3d(n_cells=16, gap_cells=1)
     d, keep = _sim_desc(w)
     keep[0][1].has_boundary = 3

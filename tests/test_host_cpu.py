@@ -122,3 +122,20 @@ def test_frictional_box_without_mu_exits_zero_with_message(tmp_path):
     assert r.returncode == 0
     assert "Need at least 1 property defined (mu_f)." in r.stderr
     assert "Job Initialized" not in r.stdout
+
+
+def test_sand_material_properties(tmp_path):
+    """Sand_SachithLocal::init: six properties (mu_2 = mu_s, I_0 = 1), eight, and fewer than
+    six exit with EXIT_FAILURE and the reference's message (sand_sachithlocal.cpp:58-117)."""
+    r = run(["-c", _write(tmp_path, mat="Sand_SachithLocal", props="1e7, 0.3, 0.01, 0.577, 1650, 1000"), "--check"],
+            cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    assert ("Material properties (E = 1e+07, nu = 0.3, G = 3.84615e+06, K = 8.33333e+06, grain diameter = 0.01, "
+            "mu_s = 0.577, grains_rho = 1650, rho_critical = 1000).") in r.stdout
+    r = run(["-c", _write(tmp_path, mat="Sand_SachithLocal", props="1e7, 0.3, 0.01, 0.577, 1650, 1000, 0.9, 0.3"),
+             "--check"], cwd=str(tmp_path))
+    assert r.returncode == 0 and "mu_2 = 0.9, i_0 = 0.3)." in r.stdout
+    r = run(["-c", _write(tmp_path, mat="Sand_SachithLocal", props="1e7, 0.3, 0.01, 0.577, 1650"), "--check"],
+            cwd=str(tmp_path))
+    assert r.returncode == 1
+    assert "Need at least 6 properties defined (E, nu, grain diameter, mu_s, grains_rho, rho_critical)." in r.stderr

@@ -692,3 +692,102 @@ def test_frictional_box_decelerates_block_near_mu_g(dim):
     np.testing.assert_allclose(v[tang] / np.linalg.norm(v[tang]), v0[tang] / np.linalg.norm(v0[tang]),
                                atol=1e-12)
     assert p["x"][:, dim - 1].min() >= 0 and np.all(p["active"] == 1)
+
+
+# ---------------------------------------------------------------- Sand_SachithLocal
+
+SAND_E, SAND_NU = 1e7, 0.3
+SAND_G = SAND_E / (2 * (1 + SAND_NU))
+
+
+def _sand(T9, L9, props, m=1.0, v=1e-3, dt=1e-4, dp=False):
+    from oracle.pyoracle import sand_update
+    return sand_update(SAND_E, SAND_NU, props, dt, m, v, L9, T9, use_density_pressure=dp)
+
+
+def _dev_tau_p(T9):
+    T = np.asarray(T9).reshape(3, 3)
+    p = -np.trace(T) / 3
+    dev = T + p * np.eye(3)
+    return dev, np.sqrt((dev ** 2).sum() / 2), p
+
+
+def test_sand_inside_yield_is_isolin():
+    """Below the yield stress mu_s p (dense, compressed) the update is the Jaumann elastic one
+    of IsotropicLinearElasticity, to the bit, with no plastic strain rate."""
+    T0 = (-1e4 * np.eye(3)).reshape(9)
+    L = np.array([[1e-3, 2e-3, 0], [0, -1e-3, 0], [0, 0, 5e-4]]).reshape(9)
+    props = W.sand_props()
+    T, nup = _sand(T0, L, props, m=1.2, v=1e-3)
+    np.testing.assert_array_equal(T, isolin_update(SAND_E, SAND_NU, 1e-4, L, T0))
+    assert nup == 0.0
+
+
+@pytest.mark.parametrize("mu_2", [0.577, 0.9])
+def test_sand_flow_follows_mu_of_I(mu_2):
+    """Sheared past yield at pressure p: the stress keeps the trial deviator's direction and
+    the pressure, and its shear stress tau obeys the local mu(I) rheology
+    tau / p = mu_s + (mu_2 - mu_s) I / (I_0 + I), with the inertial number
+    I = gamma_p_dot d sqrt(rho_g / p) of the plastic rate the update reports, and
+    gamma_p_dot = (tau_trial - tau) / (G dt). mu_2 = mu_s is perfect plasticity: tau = mu_s p."""
+    props = W.sand_props(mu_2=mu_2, I_0=0.3)
+    d, mu_s, rho_g, rho_c, m2, I0 = props
+    p0, dt = 2e4, 1e-4
+    T0 = (-p0 * np.eye(3)).reshape(9)
+    L = np.zeros((3, 3))
+    L[0, 1] = 100.0  # simple shear, far past yield in one step
+    trial = isolin_update(SAND_E, SAND_NU, dt, L.reshape(9), T0)
+    dev_tr, tau_tr, p_tr = _dev_tau_p(trial)
+    assert tau_tr > 2 * mu_s * p_tr
+    T, nup = _sand(T0, L.reshape(9), props, m=1.2, v=1e-3, dt=dt)
+    dev, tau, p = _dev_tau_p(T)
+    np.testing.assert_allclose(p, p_tr, rtol=1e-13)
+    np.testing.assert_allclose(dev, dev_tr * (tau / tau_tr), rtol=0, atol=1e-9 * tau)
+    np.testing.assert_allclose(nup, (tau_tr - tau) / (SAND_G * dt), rtol=1e-12)
+    I = nup * d * np.sqrt(rho_g / p)
+    np.testing.assert_allclose(tau / p, mu_s + (m2 - mu_s) * I / (I0 + I), rtol=1e-12)
+    if m2 == mu_s:
+        np.testing.assert_allclose(tau, mu_s * p, rtol=1e-13)
+    assert mu_s * p < tau < tau_tr
+
+
+def test_sand_tension_and_loose_packing_carry_no_stress():
+    """A tensile trial pressure, or a density below rho_critical, leaves the point stress-free
+    and reports the whole trial shear as plastic rate tau_trial / (G dt)."""
+    props = W.sand_props()
+    dt = 1e-4
+    L = np.zeros((3, 3))
+    L[0, 1] = 1.0
+    # tension: stretching with an already tensile stress
+    T0 = (1e3 * np.eye(3)).reshape(9)
+    T, nup = _sand(T0, L.reshape(9), props, m=1.2, v=1e-3, dt=dt)
+    _, tau_tr, _ = _dev_tau_p(isolin_update(SAND_E, SAND_NU, dt, L.reshape(9), T0))
+    assert np.all(T == 0) and nup == pytest.approx(tau_tr / (SAND_G * dt), rel=1e-14)
+    # compressed but loose: density 900 < rho_critical 1000
+    T0 = (-1e4 * np.eye(3)).reshape(9)
+    T, nup = _sand(T0, L.reshape(9), props, m=0.9, v=1e-3, dt=dt)
+    assert np.all(T == 0) and nup > 0
+
+
+def test_sand_density_sets_pressure():
+    """USE_DENSITY_FOR_PRESSURE: p = K (rho / rho_critical - 1), whatever the trial stress."""
+    props = W.sand_props()
+    K = SAND_E / (3 * (1 - 2 * SAND_NU))
+    T0 = (-5e3 * np.eye(3)).reshape(9)
+    L = np.zeros(9)
+    T, _ = _sand(T0, L, props, m=1.05, v=1e-3, dp=True)
+    np.testing.assert_allclose(-np.trace(T.reshape(3, 3)) / 3, K * (1050.0 / 1000.0 - 1), rtol=1e-12)
+
+
+def test_sand_plastic_strain_accumulates():
+    """gammap is the time integral of the plastic rate: stepping the shipped 2D column one step
+    at a time, gammap after the run equals the sum of gammap_dot * dt over the steps."""
+    w = W.column_collapse_example(2)
+    ora = oracle_of(w)
+    acc = np.zeros(w.bodies[0].n)
+    for _ in range(40):
+        ora.step(w.dt, 1)
+        acc += ora.plastic(0)[1] * w.dt
+    gp, _ = ora.plastic(0)
+    np.testing.assert_allclose(gp, acc, rtol=1e-12, atol=1e-18)
+    assert np.count_nonzero(gp) > 100
