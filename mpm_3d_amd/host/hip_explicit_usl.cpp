@@ -44,6 +44,18 @@ void HipExplicitUSL::init(Job *) {
     std::cout << "Solver Initialized." << std::endl;
 }
 
+void HipThreadedAlias::init(Job *) {
+    cpdi_spec = 1;
+    contact_spec = 1;
+    num_threads = 1;
+    if (int_props.size() >= 1) cpdi_spec = int_props[0];
+    if (int_props.size() >= 2) contact_spec = int_props[1];
+    if (int_props.size() >= 3) num_threads = int_props[2];
+    printf("Solver properties (cpdi_spec = %i, contact_spec = %i, num_threads = %i).\n", cpdi_spec, contact_spec,
+           num_threads);
+    std::cout << "Solver Initialized." << std::endl;
+}
+
 void HipExplicitUSL::upload(Job *job) {
     if (job->JOB_TYPE != Job::JOB_2D && job->JOB_TYPE != Job::JOB_3D)
         refuse(__func__, "HipExplicitUSL supports TYPE = 2 and 3 only.");

@@ -204,6 +204,14 @@ struct HipExplicitUSL : Solver {
     void upload(Job *job);
 };
 
+// ParallelExplicitUSL / ThreadPoolExplicitUSL configurations: the same step on the GPU; their
+// int-properties {cpdi_spec, contact_spec, num_threads[, debug]} (parallel_explicit_usl.cpp:
+// 117-153) keep their meaning, the CPU thread count has nothing left to do
+struct HipThreadedAlias : HipExplicitUSL {
+    int num_threads = 1;
+    void init(Job *job) override;
+};
+
 struct Job {
     enum { JOB_1D = 1, JOB_2D = 2, JOB_3D = 3, JOB_2D_OOP = 4, JOB_AXISYM = 5 };
     double dt = 1e-3, t = 0, t0 = 0;

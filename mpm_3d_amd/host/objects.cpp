@@ -69,6 +69,8 @@ template <> Registry<Driver>::Registry() {
 template <> Registry<Solver>::Registry() {
     table.insert(reg<Solver, HipExplicitUSL>("HipExplicitUSL"));
     table.insert(reg<Solver, HipExplicitUSL>("ExplicitUSL"));
+    table.insert(reg<Solver, HipThreadedAlias>("ParallelExplicitUSL"));
+    table.insert(reg<Solver, HipThreadedAlias>("ThreadPoolExplicitUSL"));
 }
 template <> Registry<Grid>::Registry() { table.insert(reg<Grid, CartesianLinear>("CartesianLinear")); }
 template <> Registry<Body>::Registry() { table.insert(reg<Body, Body>("DefaultBody")); }

@@ -112,3 +112,30 @@ def test_host_runs_the_shipped_column_example(tmp_path):
     assert "SCALARS gammap double 1" in last and "SCALARS gammap_dot double 1" in last
     nodes = open(tmp_path / "output" / frames[-1].replace("fpd.", "fnd.")).read()
     assert nodes.index("SCALARS active") < nodes.index("VECTORS bc_nodal_mask")
+
+
+@pytest.mark.parametrize("cfg", ["test3D.cfg", "test3D_threadpool.cfg"])
+def test_host_runs_the_shipped_parallel_example(tmp_path, cfg):
+    """examples/parallel (the 3D sand column under ParallelExplicitUSL / ThreadPoolExplicitUSL,
+    which run the same step on the GPU) to a stop time of 0.005 s, against the oracle."""
+    src = os.path.join(W.example_dir(), "parallel")
+    for f in os.listdir(src):
+        shutil.copy(os.path.join(src, f), tmp_path)
+    os.makedirs(tmp_path / "output", exist_ok=True)
+    os.makedirs(tmp_path / "save", exist_ok=True)
+    text = (tmp_path / cfg).read_text()
+    assert "properties = {1} #stop time [s]" in text
+    (tmp_path / "run.cfg").write_text(text.replace("properties = {1} #stop time [s]", "properties = {0.005} #stop time [s]"))
+    r = subprocess.run([EXE, "-c", "run.cfg", "--write-points", "final_"], cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    w = W.column_collapse_example(3)
+    w.bodies[0].x = W.read_points(os.path.join(src, "test3D.points"), 3)[2]
+    t, steps = 0.0, 0
+    while t <= 0.005:
+        steps += 1
+        t += w.dt
+    ora = oracle_of(w)
+    ora.step(w.dt, steps)
+    x = W.read_points(str(tmp_path / "final_block.points"), 3)[2]
+    assert np.max(np.abs(x - ora.points(0)["x"])) <= 1e-9 / 20

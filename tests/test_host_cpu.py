@@ -18,14 +18,16 @@ def run(args, cwd=None):
 
 
 @pytest.mark.parametrize("name,counts", [("blockdrop2D", [64]), ("collision2D", [64, 64]), ("minimal2D", [64, 64]),
-                                         ("blockdrop3D", [512]), ("collision3D", [512, 512])])
+                                         ("blockdrop3D", [512]), ("collision3D", [512, 512]),
+                                         ("columncollapse2D", [1024]), ("columncollapse3D", [2048]),
+                                         ("parallel", [2048])])
 def test_examples_configure(name, counts):
     d = os.path.join(W.example_dir(), name)
-    cfg = [f for f in os.listdir(d) if f.endswith(".cfg")][0]
-    r = run(["-c", os.path.join(d, cfg), "--check"])
-    assert r.returncode == 0, r.stderr
-    got = [int(l.split(":")[1].split()[0]) for l in r.stdout.splitlines() if l.startswith("body ")]
-    assert got == counts
+    for cfg in sorted(f for f in os.listdir(d) if f.endswith(".cfg")):
+        r = run(["-c", os.path.join(d, cfg), "--check"])
+        assert r.returncode == 0, r.stderr
+        got = [int(l.split(":")[1].split()[0]) for l in r.stdout.splitlines() if l.startswith("body ")]
+        assert got == counts, cfg
 
 
 BASE = """# comment line
