@@ -228,9 +228,8 @@ __device__ __forceinline__ void fetch_node(const KArgs &A, const SimDev &s, long
 #endif
 // Lane order of the P2G pass inside a wavefront: 0 slot order; 1 and 2 place the point of
 // in-block dual cell (x, y, z) on lane x + 4y + 16z (1) or x + 4z + 16y (2) (2D: x + 8y,
-// y + 8x) when the
-// wavefront's points sit in distinct dual cells (else slot order), so that the lanes
-// served together by the LDS hit distinct banks.
+// y + 8x): one point of each dual cell there, any further ones on the lanes left free, so
+// that the lanes served together by the LDS hit distinct banks.
 #ifndef MPM_P2G_ORDER
 #define MPM_P2G_ORDER 2
 #endif
@@ -246,18 +245,30 @@ __device__ __forceinline__ int lane_key(const int *dual) {
     return ORDER == 1 ? a0 | a1 << KB : a1 | a0 << KB;
 }
 
-// Called by the whole wavefront: lane L's point has lane-order key `key` (-1: no point),
-// `clash` marks a point that cannot take part. Returns, for every lane, the lane whose
-// point it takes (-1: none), or its own lane for every lane when two keys coincide or any
-// lane clashed. ws: 64 ints of wavefront-private LDS.
-__device__ __forceinline__ int lane_source(int *ws, int lane, int key, bool clash) {
+// Called by the whole wavefront: lane L holds a point with lane-order key `key` (-1: no
+// point; -2: a point without a key, e.g. one leaving the domain). Returns, for every lane,
+// the lane whose point it takes (-1: none). One claimant of each key gets lane `key`; the
+// points left over (a second point of one dual cell, points without a key) fill the lanes no
+// key claimed, in lane order, so every point is taken exactly once. ws, wl: 64 ints each of
+// wavefront-private LDS.
+__device__ __forceinline__ int lane_source(int *ws, int *wl, int lane, int key) {
     ws[lane] = -1;
     __builtin_amdgcn_wave_barrier();
     if (key >= 0) ws[key] = lane;
     __builtin_amdgcn_wave_barrier();
-    if (key >= 0 && ws[key] != lane) clash = true;
-    const bool any_clash = __ballot(clash) != 0ull;
-    const int src = any_clash ? lane : ws[lane];
+    const bool left = key == -2 || (key >= 0 && ws[key] != lane);
+    const unsigned long long lm = __ballot(left);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int src = ws[lane];
+    if (lm != 0ull) {  // wavefront-uniform
+        if (left) wl[__popcll(lm & below)] = lane;
+        __builtin_amdgcn_wave_barrier();
+        const unsigned long long fm = __ballot(src < 0);
+        if (src < 0) {
+            const int r = __popcll(fm & below);
+            src = r < __popcll(lm) ? wl[r] : -1;
+        }
+    }
     __builtin_amdgcn_wave_barrier();
     return src;
 }
@@ -298,7 +309,7 @@ void k_particles(KArgs A, int want_L) {
     __shared__ float laccf[DO_P2G && FF32 ? DIM * PSTR : 1];
     __shared__ GO gpat[DO_G2P ? NOUT * PSTR : 1];
     __shared__ int bb[2 * DIM];
-    __shared__ int wslot[ORDER ? 256 : 1];  // per wavefront: the lane that holds each dual cell
+    __shared__ int wslot[ORDER ? 512 : 1];  // per wavefront: the lane holding each key; left-over points
     const int tid = threadIdx.x;
     const int nent = *A.npblk;
     const R sscale = (R)1 / (R)(1 << DIM);
@@ -370,13 +381,11 @@ void k_particles(KArgs A, int want_L) {
                 // lane's point (moving 15 doubles across lanes costs more registers than the
                 // second round trip)
                 int key = -1;
-                bool clash = false;
                 if (k < count && A.act[start + k]) {
                     int dual[DIM], oct;
-                    if (PT::template dual_cell<DIM>(A, s, start + k, dual, oct)) key = lane_key<DIM, ORDER>(dual);
-                    else clash = true;
+                    key = PT::template dual_cell<DIM>(A, s, start + k, dual, oct) ? lane_key<DIM, ORDER>(dual) : -2;
                 }
-                const int src = lane_source(&wslot[(tid >> 6) * 64], lane, key, clash);
+                const int src = lane_source(&wslot[(tid >> 6) * 64], &wslot[256 + (tid >> 6) * 64], lane, key);
                 if (src != lane) k = src >= 0 ? kw + src : count;
                 a = k < count && A.act[start + k] != 0;
                 if (a) PT::template load<DIM>(A, start + k, q, DO_P2G);
@@ -397,13 +406,11 @@ void k_particles(KArgs A, int want_L) {
                 // FP32 state: load in slot order, then move the loaded points across the
                 // wavefront into the lane order
                 int key = -1;
-                bool clash = false;
                 if (a) {
                     int dual[DIM];
-                    if (PT::template dual_of<DIM>(s, q, dual)) key = lane_key<DIM, ORDER>(dual);
-                    else clash = true;
+                    key = PT::template dual_of<DIM>(s, q, dual) ? lane_key<DIM, ORDER>(dual) : -2;
                 }
-                const int src = lane_source(&wslot[(tid >> 6) * 64], lane, key, clash);
+                const int src = lane_source(&wslot[(tid >> 6) * 64], &wslot[256 + (tid >> 6) * 64], lane, key);
                 if (__ballot(src != lane) != 0ull) {  // every lane takes part in the shuffle
                     PT::template shuffle<DIM>(q, src < 0 ? lane : src);
                     a = src >= 0;
