@@ -224,7 +224,7 @@ __device__ __forceinline__ void fetch_node(const KArgs &A, const SimDev &s, long
 #define MPM_PF_G2P 1
 #endif
 #ifndef MPM_PF_P2G
-#define MPM_PF_P2G 0
+#define MPM_PF_P2G 1
 #endif
 // Lane order of the P2G pass inside a wavefront: 0 slot order; 1 and 2 place the point of
 // in-block dual cell (x, y, z) on lane x + 4y + 16z (1) or x + 4z + 16y (2) (2D: x + 8y,
@@ -313,9 +313,13 @@ void k_particles(KArgs A, int want_L) {
     const int tid = threadIdx.x;
     const int nent = *A.npblk;
     const R sscale = (R)1 / (R)(1 << DIM);
+    // the next particle block's entry is read one block ahead
+    PBlk ent_next = {0, 0, 0, 0};
+    if ((int)blockIdx.x < nent) ent_next = A.plist[blockIdx.x];
 #pragma unroll 1
     for (int e = blockIdx.x; e < nent; e += gridDim.x) {
-        const PBlk ent = A.plist[e];
+        const PBlk ent = ent_next;
+        if (e + (int)gridDim.x < nent) ent_next = A.plist[e + gridDim.x];
         const int body = uniform(ent.body);
         const int start = uniform(ent.start), count = uniform(ent.count);
         const BodyDev &B = A.bodies[body];
@@ -360,15 +364,17 @@ void k_particles(KArgs A, int want_L) {
         }
         // with PREFETCH the next point's state is loaded before this one is processed, so
         // its loads are in flight during the LDS work (different slots: no ordering hazard).
-        // (Carrying the prefetch over to the next particle block costs the G2P pass more
-        // registers than it saves in latency.)
+        // (Carrying the prefetch over to the next particle block was measured: no gain in
+        // P2G, and more registers than it saves in latency in G2P.)
         St qn;
         bool an = false;
         // the volume is needed by P2G and by the sand's density (G2P of a sand body)
         const bool need_vol = DO_P2G || B.has_material == 2;
+        // a slot's state is read whether or not the point is active, so that the reads of
+        // the activity flag and of the state are in flight together
         if (PREFETCH && tid < count) {
             an = A.act[start + tid] != 0;
-            if (an) PT::template load<DIM>(A, start + tid, qn, need_vol);
+            PT::template load<DIM>(A, start + tid, qn, need_vol);
         }
         const int lane = tid & 63;
 #pragma unroll 1
@@ -381,13 +387,16 @@ void k_particles(KArgs A, int want_L) {
                 // lane's point (moving 15 doubles across lanes costs more registers than the
                 // second round trip)
                 int key = -1;
-                if (k < count && A.act[start + k]) {
+                if (k < count) {
+                    const bool act = A.act[start + k] != 0;
                     int dual[DIM], oct;
-                    key = PT::template dual_cell<DIM>(A, s, start + k, dual, oct) ? lane_key<DIM, ORDER>(dual) : -2;
+                    const bool in = PT::template dual_cell<DIM>(A, s, start + k, dual, oct);
+                    if (act) key = in ? lane_key<DIM, ORDER>(dual) : -2;
                 }
+                // every lane given a source takes an active point
                 const int src = lane_source(&wslot[(tid >> 6) * 64], &wslot[256 + (tid >> 6) * 64], lane, key);
-                if (src != lane) k = src >= 0 ? kw + src : count;
-                a = k < count && A.act[start + k] != 0;
+                a = src >= 0;
+                k = a ? kw + src : count;
                 if (a) PT::template load<DIM>(A, start + k, q, DO_P2G);
             } else if (PREFETCH) {
                 a = an;
@@ -396,11 +405,14 @@ void k_particles(KArgs A, int want_L) {
                 an = false;
                 if (kn < count) {
                     an = A.act[start + kn] != 0;
-                    if (an) PT::template load<DIM>(A, start + kn, qn, need_vol);
+                    PT::template load<DIM>(A, start + kn, qn, need_vol);
                 }
             } else {
-                a = k < count && A.act[start + k] != 0;
-                if (a) PT::template load<DIM>(A, start + k, q, need_vol);
+                a = false;
+                if (k < count) {
+                    a = A.act[start + k] != 0;
+                    PT::template load<DIM>(A, start + k, q, need_vol);
+                }
             }
             if (ORDER && PT::FP32) {
                 // FP32 state: load in slot order, then move the loaded points across the
