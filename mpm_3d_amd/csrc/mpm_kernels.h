@@ -233,6 +233,11 @@ __device__ __forceinline__ void fetch_node(const KArgs &A, const SimDev &s, long
 #ifndef MPM_P2G_ORDER
 #define MPM_P2G_ORDER 2
 #endif
+// P2G skips the LDS atomics of window slots whose weight and gradient are exact zeros (adding
+// +-0 to an accumulator that starts at +0 changes nothing, so the sums are the same).
+#ifndef MPM_P2G_SKIP0
+#define MPM_P2G_SKIP0 1
+#endif
 // Lane of the P2G lane order for a point of in-block dual cell `dual` (KB = 2 or 3 bits per
 // dimension, 64 keys).
 template <int DIM, int ORDER>
@@ -291,6 +296,7 @@ template <int DIM, bool DO_G2P, bool DO_P2G, bool NRM, class PT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PassTune<DO_G2P, DO_P2G, PT::FP32>::waves)))
 void k_particles(KArgs A, int want_L) {
     constexpr int ORDER = (!DO_G2P && DO_P2G) ? MPM_P2G_ORDER : 0;
+    constexpr bool SKIP0 = MPM_P2G_SKIP0 != 0;
     constexpr bool PREFETCH = PassTune<DO_G2P, DO_P2G, PT::FP32>::prefetch;
     using R = typename PT::R;
     using GO = typename PT::GO;
@@ -572,6 +578,10 @@ void k_particles(KArgs A, int want_L) {
                     for (int c2 = 0; c2 < NZ; c2++) {
 #pragma unroll
                         for (int c1 = 0; c1 < 3; c1++) {
+                            // window slots no corner reaches hold exact zeros (weight and
+                            // gradient): their rows and nodes add nothing and are skipped
+                            if (SKIP0 && !(w.sw[1][c1] != (R)0 || w.dw[1][c1] != (R)0)) continue;
+                            if (SKIP0 && DIM == 3 && !(w.sw[DIM - 1][c2] != (R)0 || w.dw[DIM - 1][c2] != (R)0)) continue;
                             const RowW<D> rw = row_weights<DIM, R, D>(w, c1, c2, (D)sscale, gscale);
                             const D am = dm * rw.Su;
                             D ap[DIM], af0[DIM], af1[DIM];
@@ -588,6 +598,7 @@ void k_particles(KArgs A, int want_L) {
                             for (int c0 = 0; c0 < 3; c0++) {
                                 const int i = i0 + c0 + P * c1 + ZS * c2;
                                 const D sx = sx0[c0], dx = dx0[c0];
+                                if (SKIP0 && sx == 0.0 && dx == 0.0) continue;
                                 atomicAdd(&lacc[GF<DIM>::M * PSTR + i], am * sx);
 #pragma unroll
                                 for (int d = 0; d < DIM; d++) {
