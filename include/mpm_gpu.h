@@ -198,7 +198,9 @@ int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
  *                          first steps and keeps the fastest; those steps count
  *   MPM_OPT_PASS_GRID      workgroups of the persistent G2P and fused particle passes
  *                          (default 1024); setting it sets MPM_OPT_P2G_GRID too
- *   MPM_OPT_P2G_GRID       workgroups of the persistent P2G pass (default 1536)
+ *   MPM_OPT_P2G_GRID       workgroups of the persistent P2G pass (default: as many as the
+ *                          device holds at once for that pass, e.g. 1024 for the 3D
+ *                          precision form and 768 for 3D FP64 on MI355X)
  *   MPM_OPT_GRID_WGS       workgroups of the grid update (default 2048)
  *   MPM_OPT_GRAPH_CAPTURES (get only) graphs captured so far
  *   MPM_OPT_NODE_RECORD    1: every grid update also records nodal mass, momentum and force

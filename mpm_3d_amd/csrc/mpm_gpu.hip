@@ -248,7 +248,7 @@ struct mpm_gpu_handle {
     int buf = 0;  // which of the two particle buffers is current (toggled by every sort)
     int sort_interval = 16;
     int pass_grid = 1024;  // persistent workgroups of the G2P and fused particle passes
-    int p2g_grid = 1536;   // persistent workgroups of the P2G-only pass
+    int p2g_grid = 1536;   // persistent workgroups of the P2G-only pass (set from occupancy at create)
     int grid_wgs = 2048;   // workgroups of the grid update
     bool autotune = false, tuned = false;
     bool use_graph = false;
@@ -629,6 +629,20 @@ int launch_step(mpm_gpu_handle *h, int nsteps) {
     return MPM_OK;
 }
 
+// Workgroups of the P2G pass the device holds at once (its occupancy per CU x CUs): one
+// resident round of persistent workgroups. Measured on the 16M column: a grid of 4/3 of
+// the resident count costs 15-25% over one or two full rounds. 0 when a query fails.
+template <int DIM, class PT>
+int p2g_resident(bool nrm) {
+    int per_cu = 0, dev = 0, cus = 0;
+    const void *k = nrm ? reinterpret_cast<const void *>(&k_particles<DIM, false, true, true, PT>)
+                        : reinterpret_cast<const void *>(&k_particles<DIM, false, true, false, PT>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess) return 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return per_cu * cus;
+}
+
 int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     if (precision != MPM_PREC_FP64 && precision != MPM_PREC_FP32)
         return fail(MPM_ERR_ARG, "unknown precision form %d", precision);
@@ -659,6 +673,12 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     const Layout &L = h->lo;
     const int dim = L.dim;
     const bool f32 = precision == MPM_PREC_FP32;
+    {
+        const int r = dim == 3 ? (f32 ? p2g_resident<3, Prec32>(L.any_b1) : p2g_resident<3, Prec64>(L.any_b1))
+                               : (f32 ? p2g_resident<2, Prec32>(L.any_b1) : p2g_resident<2, Prec64>(L.any_b1));
+        if (r > 0) h->p2g_grid = r;
+        (void)hipGetLastError();  // a failed query leaves the default and no pending error
+    }
     const int nf = f32 ? (dim == 3 ? PW<3>::N : PW<2>::N) : (dim == 3 ? PF<3>::N : PF<2>::N);
     const int nacc = dim == 3 ? GF<3>::NACC : GF<2>::NACC;
     const int nout = dim == 3 ? GF<3>::NOUT : GF<2>::NOUT;

@@ -224,7 +224,7 @@ __device__ __forceinline__ void fetch_node(const KArgs &A, const SimDev &s, long
 #define MPM_PF_G2P 1
 #endif
 #ifndef MPM_PF_P2G
-#define MPM_PF_P2G 1
+#define MPM_PF_P2G 0
 #endif
 // Lane order of the P2G pass inside a wavefront: 0 slot order; 1 and 2 place the point of
 // in-block dual cell (x, y, z) on lane x + 4y + 16z (1) or x + 4z + 16y (2) (2D: x + 8y,
@@ -388,10 +388,12 @@ void k_particles(KArgs A, int want_L) {
             int k = ORDER ? kw + lane : kw;
             St q;
             bool a;
-            if (ORDER && !PT::FP32) {
-                // FP64 state: choose the lane order from the dual cells first, then load each
-                // lane's point (moving 15 doubles across lanes costs more registers than the
-                // second round trip)
+            if (ORDER) {
+                // choose the lane order from the dual cells first, then load each lane's
+                // point: a second round trip, but no state held across the permutation
+                // (loading in slot order and moving the state across lanes with __shfl
+                // costs the FP32 pass 22 VGPRs and one occupancy step; the FP64 state would
+                // cost more)
                 int key = -1;
                 if (k < count) {
                     const bool act = A.act[start + k] != 0;
@@ -418,21 +420,6 @@ void k_particles(KArgs A, int want_L) {
                 if (k < count) {
                     a = A.act[start + k] != 0;
                     PT::template load<DIM>(A, start + k, q, need_vol);
-                }
-            }
-            if (ORDER && PT::FP32) {
-                // FP32 state: load in slot order, then move the loaded points across the
-                // wavefront into the lane order
-                int key = -1;
-                if (a) {
-                    int dual[DIM];
-                    key = PT::template dual_of<DIM>(s, q, dual) ? lane_key<DIM, ORDER>(dual) : -2;
-                }
-                const int src = lane_source(&wslot[(tid >> 6) * 64], &wslot[256 + (tid >> 6) * 64], lane, key);
-                if (__ballot(src != lane) != 0ull) {  // every lane takes part in the shuffle
-                    PT::template shuffle<DIM>(q, src < 0 ? lane : src);
-                    a = src >= 0;
-                    k = kw + (src < 0 ? lane : src);
                 }
             }
             if (!a) continue;

@@ -125,34 +125,6 @@ struct Prec64 {
         return true;
     }
 
-    // dual_cell's dual cell from a loaded state
-    template <int DIM>
-    __device__ __forceinline__ static bool dual_of(const SimDev &s, const State<DIM> &q, int *dual) {
-        if (!in_domain<DIM>(s, q.x)) return false;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            const double t = q.x[d] * s.invh[d] - 0.5;
-            const int c = (int)floor(t) + 1;
-            dual[d] = c < 0 ? 0 : (c > s.N[d] ? s.N[d] : c);
-        }
-        return true;
-    }
-
-    // q becomes the state held by lane src of the wavefront
-    template <int DIM>
-    __device__ __forceinline__ static void shuffle(State<DIM> &q, int src) {
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            q.x[d] = __shfl(q.x[d], src);
-            q.v[d] = __shfl(q.v[d], src);
-        }
-#pragma unroll
-        for (int c = 0; c < PF<DIM>::NTS; c++) q.T6[c] = __shfl(q.T6[c], src);
-        q.V = __shfl(q.V, src);
-        q.m = __shfl(q.m, src);
-        q.ext = __shfl(q.ext, src);
-    }
-
     template <int DIM>
     __device__ __forceinline__ static void position(const KArgs &A, const SimDev &s, long long p, double *x,
                                                     double &ext) {
@@ -427,39 +399,6 @@ struct Prec32 {
             oct |= (upper ? 0 : 1) << d;
         }
         return true;
-    }
-
-    // dual_cell's dual cell from a loaded state
-    template <int DIM>
-    __device__ __forceinline__ static bool dual_of(const SimDev &s, const State<DIM> &q, int *dual) {
-        bool in = true;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) in = in && inside(q.c[d], q.r[d], s.N[d]);
-        if (!in) return false;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            const int dc = q.c[d] + (q.q[d] >= 0x80000000u ? 1 : 0);
-            dual[d] = dc < 0 ? 0 : (dc > s.N[d] ? s.N[d] : dc);
-        }
-        return true;
-    }
-
-    // q becomes the state held by lane src of the wavefront
-    template <int DIM>
-    __device__ __forceinline__ static void shuffle(State<DIM> &q, int src) {
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            q.c[d] = __shfl(q.c[d], src);
-            q.q[d] = __shfl(q.q[d], src);
-            q.r[d] = __shfl(q.r[d], src);
-            q.v[d] = __shfl(q.v[d], src);
-        }
-#pragma unroll
-        for (int c = 0; c < PW<DIM>::NTS; c++) q.T6[c] = __shfl(q.T6[c], src);
-        q.lnJ = __shfl(q.lnJ, src);
-        q.V = __shfl(q.V, src);
-        q.m = __shfl(q.m, src);
-        q.ext = __shfl(q.ext, src);
     }
 
     template <int DIM>
