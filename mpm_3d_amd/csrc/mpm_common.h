@@ -182,9 +182,25 @@ struct KArgs {
     double *rec;
     double *crec;
     unsigned long long *stats;  // [0] active nodes of the last grid update
+    // points the split particle passes leave to their side kernels (window not inside the
+    // block's LDS patch, or the per-tuple form): slot list (np) and counters
+    // [0] P2G count, [1] G2P count, [2] [3] their side kernels' finished workgroups
+    unsigned *rare;
+    unsigned *rare_n;
 };
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// body of point index o (bodies hold consecutive index ranges starting at p_off)
+__device__ __forceinline__ int body_of_index(const KArgs &A, long long o) {
+    int lo = 0, hi = A.nbody - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (A.bodies[mid].p_off <= o) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
 
 // sim-local node (i,j,k), -1 <= i <= N+1, -> offset inside the sim's node-block storage.
 // Storage is shifted by two nodes: the zero-weight slots of a 3-node window that hang

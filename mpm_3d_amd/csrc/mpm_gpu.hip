@@ -201,6 +201,7 @@ long long footprint_bytes(const Layout &Lo, int precision) {
     b += Lo.nblk_total * (4 + 2 * 4);      // node-block flags + two lists
     b += Lo.nkeys * (4 + 4);               // histogram, offsets
     b += Lo.npblk_total * sizeof(PBlk);    // block list
+    b += Lo.np * 4 + 16;                   // side list of the particle passes
     return b;
 }
 
@@ -237,6 +238,7 @@ struct mpm_gpu_handle {
     int *npblk = nullptr;
     double *Lout = nullptr;
     unsigned long long *stats = nullptr;
+    unsigned *rare = nullptr, *rare_n = nullptr;  // side list of the split particle passes
     bool record = false;      // MPM_OPT_NODE_RECORD
     double *rec = nullptr;    // node record planes (KArgs::rec)
     double *crec = nullptr;   // contact record planes (KArgs::crec)
@@ -299,6 +301,8 @@ KArgs kargs(mpm_gpu_handle *h) {
     A.par = h->par;
     A.Lout = h->Lout;
     A.stats = h->stats;
+    A.rare = h->rare;
+    A.rare_n = h->rare_n;
     A.rec = nullptr;  // set by launch_grid when recording
     A.crec = nullptr;
     return A;
@@ -401,6 +405,9 @@ int launch_sort(mpm_gpu_handle *h) {
     return timer_end(h, t);
 }
 
+// workgroups of the side kernels (grid-stride over a device-side count)
+constexpr int RARE_WGS = 256;
+
 template <int DIM, class PT, bool G2P, bool P2G>
 void launch_particles(mpm_gpu_handle *h, int want_L) {
     KArgs A = kargs(h);
@@ -409,6 +416,16 @@ void launch_particles(mpm_gpu_handle *h, int want_L) {
         hipLaunchKernelGGL((k_particles<DIM, G2P, P2G, true, PT>), grid, dim3(256), 0, h->stream, A, want_L);
     else
         hipLaunchKernelGGL((k_particles<DIM, G2P, P2G, false, PT>), grid, dim3(256), 0, h->stream, A, want_L);
+    // split passes leave the points outside their LDS path to a side kernel (side_pass)
+    if (!side_pass<G2P, P2G, PT::FP32>()) return;
+    if (P2G && !G2P) {
+        if (h->lo.any_b1)
+            hipLaunchKernelGGL((k_p2g_side<DIM, true, PT>), dim3(RARE_WGS), dim3(256), 0, h->stream, A);
+        else
+            hipLaunchKernelGGL((k_p2g_side<DIM, false, PT>), dim3(RARE_WGS), dim3(256), 0, h->stream, A);
+    } else if (G2P && !P2G) {
+        hipLaunchKernelGGL((k_g2p_side<DIM, PT>), dim3(RARE_WGS), dim3(256), 0, h->stream, A, want_L);
+    }
 }
 
 template <int DIM, class PT>
@@ -709,6 +726,8 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     TRY(dalloc(&h->plist, L.npblk_total));
     TRY(dalloc(&h->npblk, 1));
     TRY(dalloc(&h->stats, 4));
+    TRY(dalloc(&h->rare, L.np));
+    TRY(dalloc(&h->rare_n, 4));
     auto alloc_scan_tmp = [&](long long n, std::vector<unsigned *> &tmp) -> int {
         for (; n > 4096;) {
             const long long nb = ceil_div(n, 4096);
@@ -741,6 +760,7 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
                 : hipMemsetAsync(h->P, 0, sizeof(double) * std::max(1ll, L.np * nf), h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->act, 0, std::max(1ll, L.np), h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->stats, 0, sizeof(unsigned long long) * 4, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->rare_n, 0, sizeof(unsigned) * 4, h->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(h->d_sims, L.sims.data(), sizeof(SimDev) * L.sims.size(), hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess)
@@ -1585,7 +1605,7 @@ int mpm_gpu_destroy(mpm_gpu_handle *h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void *ptrs[] = {h->d_sims, h->d_bodies, h->d_contacts, h->P, h->P2, h->Pw, h->Pw2, h->act, h->act2, h->pid,
                     h->pid2, h->pkey, h->prank, h->acc, h->out, h->out32, h->nflag, h->nlist, h->ncount, h->bcount, h->boff,
-                    h->plist, h->npblk, h->Lout, h->stats, h->stage, h->rec, h->crec};
+                    h->plist, h->npblk, h->Lout, h->stats, h->stage, h->rec, h->crec, h->rare, h->rare_n};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (unsigned *p : h->scan_tmp)

@@ -205,6 +205,47 @@ __device__ __forceinline__ void fetch_node(const KArgs &A, const SimDev &s, long
     }
 }
 
+// Whether a split pass leaves the points outside its LDS path (window not inside the
+// block's patch, or the per-tuple form) to its side kernel (k_p2g_side / k_g2p_side). That
+// keeps those paths' registers out of the pass: measured on the 16M column, P2G FP64 gets
+// 4 waves per SIMD instead of 3 (1.50 -> 1.39 ms + 0.06 ms side kernel) and G2P FP32 sheds
+// its spills (0.90 -> 0.83 ms + 0.02 ms). Elsewhere it only costs the side kernel: the FP32
+// P2G is at 4 waves per SIMD either way, and the FP64 G2P at 2 either way.
+template <bool DO_G2P, bool DO_P2G, bool FP32>
+constexpr bool side_pass() {
+    return DO_G2P != DO_P2G && (DO_G2P ? FP32 : !FP32);
+}
+
+// Appends slot p to a side list with one atomic per wavefront (called by the lanes that
+// append, inside their branch). Each slot is appended at most once per pass, so a list of
+// np entries cannot overflow.
+__device__ __forceinline__ void side_push(unsigned *list, unsigned *count, long long p) {
+    const unsigned long long m = __ballot(1);
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    list[base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = (unsigned)p;
+}
+
+// End of a side kernel: the last workgroup to finish clears count k for the next pass.
+// Every workgroup read the count before it signs off here, so none still needs it, and
+// the clearing is a kernel store (graph replays stay correct).
+__device__ __forceinline__ void side_done(unsigned *rare_n, int k) {
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(&rare_n[2 + k], 1u) == gridDim.x - 1 ? 1 : 0;
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        rare_n[k] = 0;
+        rare_n[2 + k] = 0;
+    }
+}
+
 // Occupancy hints (minimum waves per SIMD the register allocation must allow) and
 // software prefetch of the next point's state, per pass and form. Compile-time tuning
 // knobs; the defaults are the measured best.
@@ -212,7 +253,7 @@ __device__ __forceinline__ void fetch_node(const KArgs &A, const SimDev &s, long
 #define MPM_WPE_G2P64 2
 #endif
 #ifndef MPM_WPE_P2G64
-#define MPM_WPE_P2G64 2
+#define MPM_WPE_P2G64 4
 #endif
 #ifndef MPM_WPE_G2P32
 #define MPM_WPE_G2P32 4
@@ -297,6 +338,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PassTune<DO
 void k_particles(KArgs A, int want_L) {
     constexpr int ORDER = (!DO_G2P && DO_P2G) ? MPM_P2G_ORDER : 0;
     constexpr bool SKIP0 = MPM_P2G_SKIP0 != 0;
+    constexpr bool SIDE = side_pass<DO_G2P, DO_P2G, PT::FP32>();
     constexpr bool PREFETCH = PassTune<DO_G2P, DO_P2G, PT::FP32>::prefetch;
     using R = typename PT::R;
     using GO = typename PT::GO;
@@ -432,6 +474,10 @@ void k_particles(KArgs A, int want_L) {
                     A.act[p] = 0;
                     continue;
                 }
+                if (SIDE && (mode == MODE_TUP || (mode == MODE_SEP && !window_in_patch<DIM>(w, org)))) {
+                    side_push(A.rare, &A.rare_n[1], p);  // k_g2p_side updates it
+                    continue;
+                }
                 R up[DIM], dvp[DIM], Lp[DIM][DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; d++) {
@@ -476,7 +522,7 @@ void k_particles(KArgs A, int want_L) {
                             __builtin_amdgcn_sched_barrier(0);
                         }
                     }
-                } else if (mode == MODE_SEP) {
+                } else if (!SIDE && mode == MODE_SEP) {
                     for_each_sep<DIM, R>(w, q.ext, [&](const int *off, R S, const R *G) {
                         int nijk[DIM];
 #pragma unroll
@@ -485,7 +531,7 @@ void k_particles(KArgs A, int want_L) {
                         fetch_node<DIM, PT>(A, s, gbase, nijk, vn, dvn);
                         g2p_acc<DIM, R>(vn, dvn, S, G, dtr, up, dvp, Lp);
                     });
-                } else if (mode == MODE_TUP) {
+                } else if (!SIDE && mode == MODE_TUP) {
                     PT::template tuples<DIM>(s, q, cpdi, [&](const int *nijk, R S, const R *G) {
                         R vn[DIM], dvn[DIM];
                         if (node_in_patch<DIM>(nijk, org)) {
@@ -523,6 +569,10 @@ void k_particles(KArgs A, int want_L) {
                     continue;
                 }
                 if (mode == MODE_NONE) continue;
+                if (SIDE && (mode == MODE_TUP || (mode == MODE_SEP && !window_in_patch<DIM>(w, org)))) {
+                    side_push(A.rare, &A.rare_n[0], p);  // k_p2g_side scatters it
+                    continue;
+                }
                 P2GVals<DIM, R> pv;
                 p2g_payload<DIM, R>(q, pv);
                 auto scatter_global = [&](const int *nijk, R S, const R *G) {
@@ -602,7 +652,7 @@ void k_particles(KArgs A, int want_L) {
                             }
                         }
                     }
-                } else if (mode == MODE_SEP) {
+                } else if (!SIDE && mode == MODE_SEP) {
                     int lo[DIM], hi[DIM];
 #pragma unroll
                     for (int d = 0; d < DIM; d++) {
@@ -616,7 +666,7 @@ void k_particles(KArgs A, int want_L) {
                         for (int d = 0; d < DIM; d++) nijk[d] = w.base[d] + off[d];
                         scatter_global(nijk, S, G);
                     });
-                } else {  // MODE_TUP
+                } else if (!SIDE) {  // MODE_TUP
                     PT::template tuples<DIM>(s, q, cpdi, [&](const int *nijk, R S, const R *G) {
                         if (node_in_patch<DIM>(nijk, org)) {
 #pragma unroll
@@ -687,6 +737,114 @@ void k_particles(KArgs A, int want_L) {
         }
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------------------
+// Side kernels of the split passes: the points the LDS path left (window not inside the
+// block's patch, or the per-tuple form: CPDI off, a corner outside the domain, corners over
+// more than two elements), grid-stride over the device-side count, straight to and from
+// global memory. Same arithmetic as the inline paths of the fused pass.
+
+// P2G of the listed points: global FP64 atomics, node blocks marked for the grid update.
+template <int DIM, bool NRM, class PT>
+__global__ __launch_bounds__(256) void k_p2g_side(KArgs A) {
+    using R = typename PT::R;
+    using St = typename PT::template State<DIM>;
+    constexpr int NACC = NRM ? GF<DIM>::NACC : 1 + 2 * DIM;
+    const unsigned n = A.rare_n[0];
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const long long p = A.rare[i];
+        const BodyDev &B = A.bodies[body_of_index(A, A.pid[p])];
+        const SimDev &s = A.sims[B.sim];
+        const bool b1 = NRM && B.is_b1 != 0;
+        const long long gbase = B.grid_base;
+        St q;
+        PT::template load<DIM>(A, p, q, true);
+        Win<DIM, R> w;
+        bool cpdi;
+        const int mode = PT::template window<DIM>(s, q, w, cpdi);
+        P2GVals<DIM, R> pv;
+        p2g_payload<DIM, R>(q, pv);
+        auto scatter = [&](const int *nijk, R S, const R *G) {
+            R vals[GF<DIM>::NACC];
+            p2g_node_values<DIM, R>(pv, S, G, vals, b1);
+            const long long gi = gbase + node_addr<DIM>(s, nijk);
+            for (int f = 0; f < NACC; f++) {
+                if (f >= 1 + 2 * DIM && !b1) continue;
+                unsafeAtomicAdd(&A.acc[f * A.nt + gi], (double)vals[f]);
+            }
+        };
+        if (mode == MODE_SEP) {
+            int lo[DIM], hi[DIM];
+            for (int d = 0; d < DIM; d++) {
+                lo[d] = w.base[d];
+                hi[d] = w.base[d] + 2;
+            }
+            mark_range<DIM>(A, s, lo, hi);
+            for_each_sep<DIM, R>(w, q.ext, [&](const int *off, R S, const R *G) {
+                int nijk[DIM];
+                for (int d = 0; d < DIM; d++) nijk[d] = w.base[d] + off[d];
+                scatter(nijk, S, G);
+            });
+        } else if (mode == MODE_TUP) {
+            PT::template tuples<DIM>(s, q, cpdi, [&](const int *nijk, R S, const R *G) {
+                mark_range<DIM>(A, s, nijk, nijk);
+                scatter(nijk, S, G);
+            });
+        }
+    }
+    side_done(A.rare_n, 0);
+}
+
+// G2P of the listed points: gather from the global grid output, then movePoints, strain
+// rate, density and stress update, and the stores, as in k_particles.
+template <int DIM, class PT>
+__global__ __launch_bounds__(256) void k_g2p_side(KArgs A, int want_L) {
+    using R = typename PT::R;
+    using St = typename PT::template State<DIM>;
+    const unsigned n = A.rare_n[1];
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const long long p = A.rare[i];
+        const BodyDev &B = A.bodies[body_of_index(A, A.pid[p])];
+        const SimDev &s = A.sims[B.sim];
+        const long long gbase = B.grid_base;
+        const double dt = s.dt;
+        const R dtr = (R)dt;
+        St q;
+        PT::template load<DIM>(A, p, q, B.has_material == 2);
+        Win<DIM, R> w;
+        bool cpdi;
+        const int mode = PT::template window<DIM>(s, q, w, cpdi);
+        R up[DIM], dvp[DIM], Lp[DIM][DIM];
+        for (int d = 0; d < DIM; d++) {
+            up[d] = dvp[d] = 0;
+            for (int c = 0; c < DIM; c++) Lp[d][c] = 0;
+        }
+        auto gather = [&](const int *nijk, R S, const R *G) {
+            R vn[DIM], dvn[DIM];
+            fetch_node<DIM, PT>(A, s, gbase, nijk, vn, dvn);
+            g2p_acc<DIM, R>(vn, dvn, S, G, dtr, up, dvp, Lp);
+        };
+        if (mode == MODE_SEP) {
+            for_each_sep<DIM, R>(w, q.ext, [&](const int *off, R S, const R *G) {
+                int nijk[DIM];
+                for (int d = 0; d < DIM; d++) nijk[d] = w.base[d] + off[d];
+                gather(nijk, S, G);
+            });
+        } else if (mode == MODE_TUP) {
+            PT::template tuples<DIM>(s, q, cpdi, gather);
+        }
+        PT::template advance<DIM>(B, s, dt, q, up, dvp, Lp);
+        PT::template store<DIM>(A, p, q);
+        if (B.has_material == 2) PT::template store_plastic<DIM>(A, p, q.nup, dt);
+        if (want_L && A.Lout) {
+            double L9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int r = 0; r < DIM; r++)
+                for (int c = 0; c < DIM; c++) L9[3 * r + c] = Lp[r][c];
+            for (int kk = 0; kk < 9; kk++) A.Lout[kk * A.np + p] = L9[kk];
+        }
+    }
+    side_done(A.rare_n, 1);
 }
 
 // ---------------------------------------------------------------------------------------

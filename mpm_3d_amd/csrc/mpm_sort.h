@@ -22,16 +22,6 @@
 
 namespace mpm {
 
-__device__ __forceinline__ int body_of_index(const KArgs &A, long long o) {
-    int lo = 0, hi = A.nbody - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (A.bodies[mid].p_off <= o) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 __device__ __forceinline__ int body_of_block(const KArgs &A, long long key) {
     int lo = 0, hi = A.nbody - 1;
     while (lo < hi) {
