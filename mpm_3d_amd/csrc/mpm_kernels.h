@@ -414,7 +414,7 @@ void k_particles(KArgs A, int want_L) {
         // its loads are in flight during the LDS work (different slots: no ordering hazard).
         // (Carrying the prefetch over to the next particle block was measured: no gain in
         // P2G, and more registers than it saves in latency in G2P.)
-        St qn;
+        typename PT::template Raw<DIM> qn;
         bool an = false;
         // the volume is needed by P2G and by the sand's density (G2P of a sand body)
         const bool need_vol = DO_P2G || B.has_material == 2;
@@ -422,7 +422,7 @@ void k_particles(KArgs A, int want_L) {
         // the activity flag and of the state are in flight together
         if (PREFETCH && tid < count) {
             an = A.act[start + tid] != 0;
-            PT::template load<DIM>(A, start + tid, qn, need_vol);
+            PT::template load_raw<DIM>(A, start + tid, qn, need_vol);
         }
         const int lane = tid & 63;
 #pragma unroll 1
@@ -450,12 +450,12 @@ void k_particles(KArgs A, int want_L) {
                 if (a) PT::template load<DIM>(A, start + k, q, DO_P2G);
             } else if (PREFETCH) {
                 a = an;
-                q = qn;
+                PT::template cook<DIM>(qn, q, need_vol);
                 const int kn = k + blockDim.x;
                 an = false;
                 if (kn < count) {
                     an = A.act[start + kn] != 0;
-                    PT::template load<DIM>(A, start + kn, qn, need_vol);
+                    PT::template load_raw<DIM>(A, start + kn, qn, need_vol);
                 }
             } else {
                 a = false;

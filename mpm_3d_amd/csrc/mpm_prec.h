@@ -30,6 +30,15 @@ struct Prec64 {
         double nup;  // Sand_SachithLocal plastic shear rate of the last update
     };
 
+    // a prefetched state: the state itself
+    template <int DIM> using Raw = State<DIM>;
+    template <int DIM>
+    __device__ __forceinline__ static void load_raw(const KArgs &A, long long p, Raw<DIM> &w, bool need_vol) {
+        load<DIM>(A, p, w, need_vol);
+    }
+    template <int DIM>
+    __device__ __forceinline__ static void cook(const Raw<DIM> &w, State<DIM> &q, bool) { q = w; }
+
     template <int DIM>
     __device__ __forceinline__ static void load(const KArgs &A, long long p, State<DIM> &q, bool) {
         using F = PF<DIM>;
@@ -146,6 +155,48 @@ struct Prec32 {
         float v[DIM], T6[6], lnJ, V, m, ext;  // V = V0 exp(lnJ), loaded for P2G and sand only
         float nup;  // Sand_SachithLocal plastic shear rate of the last update
     };
+
+    // a prefetched state as stored (packed cell, fixed-point offsets, V0): fewer registers
+    // while the loads are in flight; cook() derives the rest
+    template <int DIM> struct Raw {
+        uint32_t cell, q[DIM];
+        float v[DIM], T[Cfg<DIM>::NTS], lnJ, V0, m, ext;
+    };
+    template <int DIM>
+    __device__ __forceinline__ static void load_raw(const KArgs &A, long long p, Raw<DIM> &w, bool need_vol) {
+        using F = PW<DIM>;
+        const float *Pf = reinterpret_cast<const float *>(A.Pw);
+        w.cell = A.Pw[F::CELL * A.np + p];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            w.q[d] = A.Pw[(F::Q + d) * A.np + p];
+            w.v[d] = Pf[(F::V + d) * A.np + p];
+        }
+#pragma unroll
+        for (int c = 0; c < F::NTS; c++) w.T[c] = Pf[(F::T + c) * A.np + p];
+        w.lnJ = Pf[F::LNJ * A.np + p];
+        w.V0 = need_vol ? Pf[F::V0 * A.np + p] : 0.0f;
+        w.m = Pf[F::MASS * A.np + p];
+        w.ext = Pf[F::EXT * A.np + p];
+    }
+    template <int DIM>
+    __device__ __forceinline__ static void cook(const Raw<DIM> &w, State<DIM> &q, bool need_vol) {
+        unpack_cell<DIM>(w.cell, q.c);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            q.q[d] = w.q[d];
+            q.r[d] = offset(w.q[d]);
+            q.v[d] = w.v[d];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; c++) q.T6[c] = 0.0f;
+#pragma unroll
+        for (int c = 0; c < Cfg<DIM>::NTS; c++) q.T6[c] = w.T[c];
+        q.lnJ = w.lnJ;
+        q.V = need_vol ? w.V0 * expf(w.lnJ) : 0.0f;
+        q.m = w.m;
+        q.ext = w.ext;
+    }
 
     template <int DIM>
     __device__ __forceinline__ static void load(const KArgs &A, long long p, State<DIM> &q, bool need_vol) {
