@@ -349,12 +349,9 @@ void k_particles(KArgs A, int want_L) {
     constexpr int NZ = DIM == 3 ? 3 : 1;
     constexpr int PBLKS = Cfg<DIM>::PBLKS;
     // P2G accumulator patch, FP64 in both forms. (Keeping the precision form's force partial
-    // sums in FP32 -- FF32 -- is supported but measured 5x slower on gfx950: FP32 LDS
-    // atomics (ds_add_f32) are much slower than ds_add_f64 there.)
-    constexpr bool FF32 = false;
-    constexpr int NACC64 = FF32 ? NACC - DIM : NACC;
-    __shared__ double lacc[DO_P2G ? NACC64 * PSTR : 1];
-    __shared__ float laccf[DO_P2G && FF32 ? DIM * PSTR : 1];
+    // sums in FP32 was measured 5x slower on gfx950: FP32 LDS atomics (ds_add_f32) are much
+    // slower than ds_add_f64 there.)
+    __shared__ double lacc[DO_P2G ? NACC * PSTR : 1];
     __shared__ GO gpat[DO_G2P ? NOUT * PSTR : 1];
     __shared__ int bb[2 * DIM];
     __shared__ int wslot[ORDER ? 512 : 1];  // per wavefront: the lane holding each key; left-over points
@@ -379,9 +376,7 @@ void k_particles(KArgs A, int want_L) {
         int org[DIM];
         patch_origin<DIM>(s, uniform(ent.key) - B.pblk_off, org);
         if (DO_P2G) {
-            for (int i = tid; i < NACC64 * PSTR; i += blockDim.x) lacc[i] = 0.0;
-            if (FF32)
-                for (int i = tid; i < DIM * PSTR; i += blockDim.x) laccf[i] = 0.0f;
+            for (int i = tid; i < NACC * PSTR; i += blockDim.x) lacc[i] = 0.0;
             if (tid < DIM) {
                 bb[tid] = 0x7fffffff;
                 bb[DIM + tid] = -0x7fffffff;
@@ -640,11 +635,10 @@ void k_particles(KArgs A, int want_L) {
 #pragma unroll
                                 for (int d = 0; d < DIM; d++) {
                                     atomicAdd(&lacc[(GF<DIM>::P + d) * PSTR + i], ap[d] * sx);
-                                    if (FF32) atomicAdd(&laccf[d * PSTR + i], (float)fma(af0[d], dx, af1[d] * sx));
-                                    else atomicAdd(&lacc[(GF<DIM>::F + d) * PSTR + i], fma(af0[d], dx, af1[d] * sx));
+                                    atomicAdd(&lacc[(GF<DIM>::F + d) * PSTR + i], fma(af0[d], dx, af1[d] * sx));
                                 }
                                 if (NRM && b1) {
-                                    constexpr int N0 = GF<DIM>::NRM - (FF32 ? DIM : 0);
+                                    constexpr int N0 = GF<DIM>::NRM;
                                     atomicAdd(&lacc[(N0 + 0) * PSTR + i], an0 * dx);
                                     atomicAdd(&lacc[(N0 + 1) * PSTR + i], an1 * sx);
                                     if (DIM == 3) atomicAdd(&lacc[(N0 + DIM - 1) * PSTR + i], an2 * sx);
@@ -679,9 +673,7 @@ void k_particles(KArgs A, int want_L) {
                             const int i = patch_index<DIM>(nijk, org);
                             for (int f = 0; f < NACC; f++) {
                                 if (f >= 1 + 2 * DIM && !b1) continue;
-                                const bool isf = f >= GF<DIM>::F && f < GF<DIM>::F + DIM;
-                                if (FF32 && isf) atomicAdd(&laccf[(f - GF<DIM>::F) * PSTR + i], (float)vals[f]);
-                                else atomicAdd(&lacc[(FF32 && f >= GF<DIM>::NRM ? f - DIM : f) * PSTR + i], (double)vals[f]);
+                                atomicAdd(&lacc[f * PSTR + i], (double)vals[f]);
                             }
                         } else {
                             mark_range<DIM>(A, s, nijk, nijk);
@@ -722,9 +714,7 @@ void k_particles(KArgs A, int want_L) {
 #pragma unroll
                 for (int f = 0; f < NACC; f++) {
                     if (f >= 1 + 2 * DIM && !b1) continue;
-                    const bool isf = f >= GF<DIM>::F && f < GF<DIM>::F + DIM;
-                    const double v = (FF32 && isf) ? (double)laccf[(f - GF<DIM>::F) * PSTR + li]
-                                                   : lacc[(FF32 && f >= GF<DIM>::NRM ? f - DIM : f) * PSTR + li];
+                    const double v = lacc[f * PSTR + li];
                     if (v != 0.0) unsafeAtomicAdd(&A.acc[f * A.nt + gi], v);
                 }
             }
