@@ -239,6 +239,7 @@ struct mpm_gpu_handle {
     double *Lout = nullptr;
     unsigned long long *stats = nullptr;
     unsigned *rare = nullptr, *rare_n = nullptr;  // side list of the split particle passes
+    int grid_nbm = MAXB;  // bodies per simulation the grid kernel is sized for (1, 2 or MAXB)
     bool record = false;      // MPM_OPT_NODE_RECORD
     double *rec = nullptr;    // node record planes (KArgs::rec)
     double *crec = nullptr;   // contact record planes (KArgs::crec)
@@ -438,7 +439,10 @@ void launch_grid(mpm_gpu_handle *h, bool want_stats) {
         Ag.crec = h->crec;
     }
     h->recorded = h->record;
-    hipLaunchKernelGGL((k_grid<DIM, typename PT::GO>), dim3(h->grid_wgs), dim3(256), 0, h->stream, Ag);
+    using GO = typename PT::GO;
+    if (h->grid_nbm == 1) hipLaunchKernelGGL((k_grid<DIM, GO, 1>), dim3(h->grid_wgs), dim3(256), 0, h->stream, Ag);
+    else if (h->grid_nbm == 2) hipLaunchKernelGGL((k_grid<DIM, GO, 2>), dim3(h->grid_wgs), dim3(256), 0, h->stream, Ag);
+    else hipLaunchKernelGGL((k_grid<DIM, GO, MAXB>), dim3(h->grid_wgs), dim3(256), 0, h->stream, Ag);
 }
 
 // sort when the order is stale: after an upload, when the interval has passed, or at the
@@ -690,6 +694,12 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     const Layout &L = h->lo;
     const int dim = L.dim;
     const bool f32 = precision == MPM_PREC_FP32;
+    {
+        // no later load_sim gives a simulation more bodies than it was created with
+        int mx = 1;
+        for (const SimDev &sd : L.sims) mx = std::max(mx, sd.nbody);
+        h->grid_nbm = mx <= 1 ? 1 : (mx <= 2 ? 2 : MAXB);
+    }
     {
         const int r = dim == 3 ? (f32 ? p2g_resident<3, Prec32>(L.any_b1) : p2g_resident<3, Prec64>(L.any_b1))
                                : (f32 ? p2g_resident<2, Prec32>(L.any_b1) : p2g_resident<2, Prec64>(L.any_b1));

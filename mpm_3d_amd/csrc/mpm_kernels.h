@@ -839,7 +839,10 @@ __global__ __launch_bounds__(256) void k_g2p_side(KArgs A, int want_L) {
 
 // ---------------------------------------------------------------------------------------
 // grid update: body force, contact, boundary, moveGrid. One wavefront per node block.
-template <int DIM, class GO>
+// NBM: bodies per simulation the kernel handles (1, 2 or MAXB), chosen by the host from the
+// largest simulation in the handle: the per-body node values live in registers, and sizing
+// them to 1 body keeps a one-body grid update at 8 waves per SIMD instead of 2.
+template <int DIM, class GO, int NBM>
 __global__ __launch_bounds__(256) void k_grid(KArgs A) {
     GO *outp = nullptr;
     if constexpr (sizeof(GO) == 8) outp = reinterpret_cast<GO *>(A.out);
@@ -894,9 +897,9 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
             const int nb = s.nbody;
             // f0: the force before contacts act, which the frictional box takes its normal
             // impulse from (cartesian_frictional_box.cpp:102-117 recomputes it from S and gradS)
-            double m[MAXB], p[MAXB][DIM], f[MAXB][DIM], f0[MAXB][DIM], nr[MAXB][DIM];
+            double m[NBM], p[NBM][DIM], f[NBM][DIM], f0[NBM][DIM], nr[NBM][DIM];
 #pragma unroll
-            for (int b = 0; b < MAXB; b++) {
+            for (int b = 0; b < NBM; b++) {
                 m[b] = 0.0;
 #pragma unroll
                 for (int d = 0; d < DIM; d++) p[b][d] = f[b][d] = f0[b][d] = nr[b][d] = 0.0;
@@ -926,11 +929,11 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
             }
             const double dt = s.dt;
             // ContactHuang::applyRules, contacts in configuration order
-            for (int c = 0; c < s.ncontact; c++) {
+            for (int c = 0; NBM >= 2 && c < s.ncontact; c++) {  // a contact needs two bodies
                 const ContactDev C = A.contacts[s.contact0 + c];
                 double m1 = 0, m2 = 0, p1[DIM] = {}, p2[DIM] = {}, f1[DIM] = {}, f2[DIM] = {}, n1[DIM] = {};
 #pragma unroll
-                for (int b = 0; b < MAXB; b++) {
+                for (int b = 0; b < NBM; b++) {
                     if (b == C.b1) {
                         m1 = m[b];
 #pragma unroll
@@ -1002,7 +1005,7 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
                 const double fa = C.mu * fabs(fn), fb = fabs(ft);
                 const double fsc = fb < fa ? fb : fa;
 #pragma unroll
-                for (int b = 0; b < MAXB; b++) {
+                for (int b = 0; b < NBM; b++) {
 #pragma unroll
                     for (int d = 0; d < DIM; d++) {
                         const double fc = nrm[d] * fnc + s1[d] * fsc;
@@ -1017,7 +1020,7 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
             }
             // the body's boundary, then moveGrid
 #pragma unroll
-            for (int b = 0; b < MAXB; b++) {
+            for (int b = 0; b < NBM; b++) {
                 if (b >= nb) continue;
                 const BodyDev &B = A.bodies[s.body0 + b];
                 if (B.has_boundary == 1 && wall) {
