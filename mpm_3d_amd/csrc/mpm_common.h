@@ -187,6 +187,8 @@ struct KArgs {
     // [0] P2G count, [1] G2P count, [2] [3] their side kernels' finished workgroups
     unsigned *rare;
     unsigned *rare_n;
+    // block counter of the particle passes ([0] next block, [1] finished workgroups)
+    unsigned *work;
 };
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }

@@ -276,6 +276,18 @@ __device__ __forceinline__ void side_done(unsigned *rare_n, int k) {
 #endif
 // P2G skips the LDS atomics of window slots whose weight and gradient are exact zeros (adding
 // +-0 to an accumulator that starts at +0 changes nothing, so the sums are the same).
+#ifndef MPM_DYN_BLOCKS
+#define MPM_DYN_BLOCKS 1
+#endif
+// Which passes take their particle blocks from a counter instead of a fixed stride. Measured
+// on the 16M column against the fixed stride: P2G 1.25 -> 1.19 ms (FP32) and 1.44 -> 1.40 ms
+// (FP64), G2P FP64 1.63 -> 1.61 ms; but G2P FP32 0.82 -> 0.85 ms (its first prefetch of a block
+// waits for the block entry) and the 2D ensemble 219 -> 188 solves/s (blocks of a few points,
+// where a counter round trip per block dominates), so those keep the fixed stride.
+template <int DIM, bool DO_G2P, bool DO_P2G, bool FP32>
+constexpr bool dyn_blocks() {
+    return DIM == 3 && DO_G2P != DO_P2G && (DO_P2G || !FP32);
+}
 #ifndef MPM_P2G_SKIP0
 #define MPM_P2G_SKIP0 1
 #endif
@@ -338,6 +350,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PassTune<DO
 void k_particles(KArgs A, int want_L) {
     constexpr int ORDER = (!DO_G2P && DO_P2G) ? MPM_P2G_ORDER : 0;
     constexpr bool SKIP0 = MPM_P2G_SKIP0 != 0;
+    constexpr bool DYN = MPM_DYN_BLOCKS != 0 && dyn_blocks<DIM, DO_G2P, DO_P2G, PT::FP32>();
     constexpr bool SIDE = side_pass<DO_G2P, DO_P2G, PT::FP32>();
     constexpr bool PREFETCH = PassTune<DO_G2P, DO_P2G, PT::FP32>::prefetch;
     using R = typename PT::R;
@@ -358,13 +371,24 @@ void k_particles(KArgs A, int want_L) {
     const int tid = threadIdx.x;
     const int nent = *A.npblk;
     const R sscale = (R)1 / (R)(1 << DIM);
-    // the next particle block's entry is read one block ahead
+    // Block order. DYN: workgroups take blocks from a counter, the next index fetched one
+    // block ahead (e_sh double-buffered: a slot is rewritten only after every thread read it
+    // behind the end-of-block barrier), so no workgroup idles while others have blocks left.
+    // Otherwise workgroup w takes blocks w, w + grid, ..., the next entry read one block ahead.
+    __shared__ int e_sh[2];
+    int e = blockIdx.x, epar = 0;
+    if (DYN) {
+        if (tid == 0) e_sh[0] = (int)atomicAdd(&A.work[0], 1u);
+        __syncthreads();
+        e = uniform(e_sh[0]);
+    }
     PBlk ent_next = {0, 0, 0, 0};
-    if ((int)blockIdx.x < nent) ent_next = A.plist[blockIdx.x];
+    if (!DYN && e < nent) ent_next = A.plist[e];
 #pragma unroll 1
-    for (int e = blockIdx.x; e < nent; e += gridDim.x) {
-        const PBlk ent = ent_next;
-        if (e + (int)gridDim.x < nent) ent_next = A.plist[e + gridDim.x];
+    while (e < nent) {
+        if (DYN && tid == 0) e_sh[epar ^ 1] = (int)atomicAdd(&A.work[0], 1u);
+        const PBlk ent = DYN ? A.plist[e] : ent_next;
+        if (!DYN && e + (int)gridDim.x < nent) ent_next = A.plist[e + gridDim.x];
         const int body = uniform(ent.body);
         const int start = uniform(ent.start), count = uniform(ent.count);
         const BodyDev &B = A.bodies[body];
@@ -726,6 +750,24 @@ void k_particles(KArgs A, int want_L) {
             }
         }
         __syncthreads();
+        if (DYN) {
+            e = uniform(e_sh[epar ^ 1]);
+            epar ^= 1;
+        } else {
+            e += gridDim.x;
+        }
+    }
+    if (DYN) {
+        // the last workgroup to finish clears the counter for the next pass: every workgroup
+        // read its final (out of range) index before it signs off here
+        __syncthreads();
+        if (tid == 0) {
+            __threadfence();
+            if (atomicAdd(&A.work[1], 1u) == gridDim.x - 1) {
+                A.work[0] = 0;
+                A.work[1] = 0;
+            }
+        }
     }
 }
 
