@@ -376,19 +376,22 @@ void k_particles(KArgs A, int want_L) {
     // behind the end-of-block barrier), so no workgroup idles while others have blocks left.
     // Otherwise workgroup w takes blocks w, w + grid, ..., the next entry read one block ahead.
     __shared__ int e_sh[2];
+    // the counter costs two contended atomics per workgroup and one per block: taken only
+    // when there are many blocks per workgroup (small simulations keep the fixed stride)
+    const bool dyn = DYN && nent >= 8 * (int)gridDim.x;
     int e = blockIdx.x, epar = 0;
-    if (DYN) {
+    if (dyn) {
         if (tid == 0) e_sh[0] = (int)atomicAdd(&A.work[0], 1u);
         __syncthreads();
         e = uniform(e_sh[0]);
     }
     PBlk ent_next = {0, 0, 0, 0};
-    if (!DYN && e < nent) ent_next = A.plist[e];
+    if (!dyn && e < nent) ent_next = A.plist[e];
 #pragma unroll 1
     while (e < nent) {
-        if (DYN && tid == 0) e_sh[epar ^ 1] = (int)atomicAdd(&A.work[0], 1u);
-        const PBlk ent = DYN ? A.plist[e] : ent_next;
-        if (!DYN && e + (int)gridDim.x < nent) ent_next = A.plist[e + gridDim.x];
+        if (dyn && tid == 0) e_sh[epar ^ 1] = (int)atomicAdd(&A.work[0], 1u);
+        const PBlk ent = dyn ? A.plist[e] : ent_next;
+        if (!dyn && e + (int)gridDim.x < nent) ent_next = A.plist[e + gridDim.x];
         const int body = uniform(ent.body);
         const int start = uniform(ent.start), count = uniform(ent.count);
         const BodyDev &B = A.bodies[body];
@@ -750,14 +753,14 @@ void k_particles(KArgs A, int want_L) {
             }
         }
         __syncthreads();
-        if (DYN) {
+        if (dyn) {
             e = uniform(e_sh[epar ^ 1]);
             epar ^= 1;
         } else {
             e += gridDim.x;
         }
     }
-    if (DYN) {
+    if (dyn) {
         // the last workgroup to finish clears the counter for the next pass: every workgroup
         // read its final (out of range) index before it signs off here
         __syncthreads();
