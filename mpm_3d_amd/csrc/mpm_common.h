@@ -314,8 +314,12 @@ __host__ __device__ __forceinline__ int point_window(const SimDev &s, const doub
                                             bool &cpdi_out) {
     cpdi_out = false;
     if (!in_domain<DIM>(s, x)) return MODE_LEFT;
-    if (s.cpdi_spec != 1) return MODE_TUP;
-    const double a = 0.5 * ext;
+    // CPDI off (cpdi_spec 0, or a corner outside the domain: default_points.cpp:220-263) is
+    // the linear basis over the point's own element: the same window with both "corners" at
+    // the point (sw = 2 (1 - |r|) on the element's two nodes) and the gradient slots set to
+    // -/+ extent / h, which the common factors turn into -/+ prod_{e != d} (1 - |r_e|) / h_d.
+    bool off = s.cpdi_spec != 1;
+    const double a = off ? 0.0 : 0.5 * ext;
     double xm[DIM], xp[DIM];
     bool cin = true;
 #pragma unroll
@@ -324,8 +328,13 @@ __host__ __device__ __forceinline__ int point_window(const SimDev &s, const doub
         xp[d] = x[d] + a * (1.0);
         cin = cin && (xm[d] <= s.L[d] && xm[d] >= 0.0) && (xp[d] <= s.L[d] && xp[d] >= 0.0);
     }
-    if (!cin) return MODE_TUP;  // a corner leaves the domain: CPDI off (default_points.cpp:220-235)
-    cpdi_out = true;
+    if (!cin) {  // a corner leaves the domain: CPDI off
+        off = true;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) xm[d] = xp[d] = x[d];
+    }
+    if (off && !(ext > 0.0)) return MODE_TUP;
+    cpdi_out = !off;
     int mode = MODE_SEP;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
@@ -363,6 +372,15 @@ __host__ __device__ __forceinline__ int point_window(const SimDev &s, const doub
             d0 += k0 ? h0 : 0.0;
             d1 += k0 ? h1 : h0;
             d2 += k0 ? 0.0 : h1;
+        }
+        if (off) {
+            // on a node line the reference's CPDI-off gradient is 0/0: per-tuple path
+            if (vm && x[d] == hd * (double)em) mode = mode == MODE_NONE ? MODE_NONE : MODE_TUP;
+            const double g = ext / hd;
+            const bool k0 = em == b;
+            d0 = k0 ? -g : 0.0;
+            d1 = k0 ? g : -g;
+            d2 = k0 ? 0.0 : g;
         }
         w.sw[d][0] = s0; w.sw[d][1] = s1; w.sw[d][2] = s2;
         w.dw[d][0] = d0; w.dw[d][1] = d1; w.dw[d][2] = d2;

@@ -254,18 +254,26 @@ struct Prec32 {
 #pragma unroll
         for (int d = 0; d < DIM; d++) in = in && inside(q.c[d], q.r[d], s.N[d]);
         if (!in) return MODE_LEFT;
-        if (s.cpdi_spec != 1) return MODE_TUP;
+        // CPDI off (cpdi_spec 0, or a corner outside the domain): both "corners" at the point
+        // and the gradient slots set to -/+ extent / h, as in point_window; on a node line this
+        // is the one-sided derivative of the per-tuple path below
+        bool off = s.cpdi_spec != 1;
         float cm[DIM], cp[DIM];
         bool cin = true;
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
-            const float a = 0.5f * q.ext * s.invhf[d];  // half extent in cells
+            const float a = off ? 0.0f : 0.5f * q.ext * s.invhf[d];  // half extent in cells
             cm[d] = q.r[d] - a;
             cp[d] = q.r[d] + a;
             cin = cin && inside(q.c[d], cm[d], s.N[d]) && inside(q.c[d], cp[d], s.N[d]);
         }
-        if (!cin) return MODE_TUP;  // a corner leaves the domain: CPDI off
-        cpdi = true;
+        if (!cin) {  // a corner leaves the domain: CPDI off
+            off = true;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) cm[d] = cp[d] = q.r[d];
+        }
+        if (off && !(q.ext > 0.0f)) return MODE_TUP;
+        cpdi = !off;
         int mode = MODE_SEP;
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
@@ -300,6 +308,13 @@ struct Prec32 {
                 d0 += k0 ? h0 : 0.0f;
                 d1 += k0 ? h1 : h0;
                 d2 += k0 ? 0.0f : h1;
+            }
+            if (off) {
+                const float g = q.ext * s.invhf[d];
+                const bool k0 = em == b;
+                d0 = k0 ? -g : 0.0f;
+                d1 = k0 ? g : -g;
+                d2 = k0 ? 0.0f : g;
             }
             w.sw[d][0] = s0; w.sw[d][1] = s1; w.sw[d][2] = s2;
             w.dw[d][0] = d0; w.dw[d][1] = d1; w.dw[d][2] = d2;
