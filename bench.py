@@ -40,8 +40,8 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-updates/s, 16M-pt 3D (FP32 vs FP64); batched solves/s + p99 latency"
 HBM_PEAK_GBS = 8000.0
 
-# algorithmic bytes per launch, 3D, as (bytes per active point, bytes per active node);
-# the per-kernel table is in DESIGN.md
+# algorithmic bytes per launch, 3D, as (bytes per active point, bytes per active node): every
+# plane the kernel must read once and every plane it must write once
 BYTES = {
     "fp64": {
         "pass": (224, 104),     # read x3 v3 T6 V m extent, write x3 v3 T6 V; read v3 dv3, write m p3 f3
