@@ -78,6 +78,22 @@ def test_sand_precision_form():
     assert e["x_cells"] <= 1e-5 and e["v_rel"] <= 1e-4 and e["T_rel"] <= 1e-4 and e["V_rel"] <= 1e-5, e
 
 
+def test_sand_precision_form_3d():
+    """The precision form on the shipped 3D column (100 steps): positions to 1e-5 h; velocities,
+    stresses and volumes to the bounds of the 2D case above (same rho_critical threshold)."""
+    from tests.test_gpu_precision import _errors, _gpu_state, _ora_state
+    w = W.column_collapse_example(3)
+    ora = oracle_of(w)
+    g = gpu_of(w, precision=MPM_PREC_FP32)
+    ora.step(w.dt, 100)
+    g.step(w.dt, 100)
+    ro, rg = _ora_state(ora, 0), _gpu_state(g, 0)
+    np.testing.assert_array_equal(rg["active"], ro["active"])
+    e = _errors(w, ro, rg, 100 * w.dt)
+    print(f"sand fp32 3D: {e}")
+    assert e["x_cells"] <= 1e-5 and e["v_rel"] <= 1e-4 and e["T_rel"] <= 1e-4 and e["V_rel"] <= 1e-5, e
+
+
 def test_host_runs_the_shipped_column_example(tmp_path):
     """mpm_v3_hip -c test2D.cfg of examples/columncollapse2D, unchanged but for the stop time
     (0.02 s): the final points match the oracle, and the DefaultVTK frames carry the sand's
