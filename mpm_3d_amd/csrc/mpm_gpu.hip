@@ -391,8 +391,8 @@ int launch_sort(mpm_gpu_handle *h) {
                        nk - 1, h->inact_count);
     scan_exclusive(h, h->bcount, h->boff, nk, h->scan_tmp);             // bucket offsets
     scan_exclusive(h, h->inact_count, h->inact_off, gp, h->scan_tmp_i);  // inactive ranks per workgroup
-    hipLaunchKernelGGL(k_make_plist, dim3((unsigned)ceil_div(nk, 256)), dim3(256), 0, h->stream, A, h->bcount,
-                       h->boff, h->plist, h->npblk, nk, 1 << DIM);
+    hipLaunchKernelGGL(k_make_plist, dim3((unsigned)ceil_div(nk, 256)), dim3(256), 0, h->stream, A, h->boff,
+                       h->plist, h->npblk, nk, 1 << DIM);
     hipLaunchKernelGGL(k_clear, dim3((unsigned)ceil_div(nk, 256)), dim3(256), 0, h->stream, h->bcount, nk);
     if constexpr (PT::FP32) {
         hipLaunchKernelGGL((k_bin_scatter<uint32_t, PW<DIM>::N>), dim3(gp), dim3(256), 0, h->stream, np, h->Pw, h->Pw2,
@@ -1558,7 +1558,8 @@ int mpm_gpu_copy_probe(int64_t nbytes, int reps, double *gbps) {
 int mpm_gpu_host_weights(const mpm_grid_desc *grid, const double *x, double extent, int32_t *node_ijk, double *S,
                          double *G, int max_tuples) {
     if (!grid || !x || !node_ijk || !S || !G) return fail(MPM_ERR_ARG, "null argument");
-    mpm_body_desc bd{1, 0, 0.0, 0.0, 0};
+    mpm_body_desc bd{};
+    bd.n_points = 1;
     mpm_sim_desc sd{};
     sd.grid = *grid;
     sd.nbody = 1;

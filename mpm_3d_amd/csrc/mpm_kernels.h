@@ -324,7 +324,7 @@ __device__ __forceinline__ int lane_source(int *ws, int *wl, int lane, int key) 
         const unsigned long long fm = __ballot(src < 0);
         if (src < 0) {
             const int r = __popcll(fm & below);
-            src = r < __popcll(lm) ? wl[r] : -1;
+            src = r < (int)__popcll(lm) ? wl[r] : -1;
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -356,7 +356,7 @@ void k_particles(KArgs A, int want_L) {
     using R = typename PT::R;
     using GO = typename PT::GO;
     using St = typename PT::template State<DIM>;
-    constexpr int PATCH = Cfg<DIM>::PATCH, P = Cfg<DIM>::P, PSTR = Cfg<DIM>::PSTR, ZS = Cfg<DIM>::ZS;
+    constexpr int P = Cfg<DIM>::P, PSTR = Cfg<DIM>::PSTR, ZS = Cfg<DIM>::ZS;
     constexpr int NACC = NRM ? GF<DIM>::NACC : 1 + 2 * DIM;
     constexpr int NOUT = GF<DIM>::NOUT;
     constexpr int NZ = DIM == 3 ? 3 : 1;

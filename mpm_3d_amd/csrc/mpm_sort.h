@@ -13,7 +13,7 @@
 //   k_bin_count   key per slot (inactive points go to the last bucket), wavefront-
 //                 aggregated atomic histogram, rank inside the bucket
 //   scan          exclusive prefix sum of the histogram (k_scan_*)
-//   k_make_plist  one PBlk per non-empty bucket; clears the histogram for the next sort
+//   k_make_plist  one PBlk per non-empty bucket; k_clear then clears the histogram
 //   k_bin_scatter moves every particle plane to its sorted slot
 #pragma once
 
@@ -129,10 +129,10 @@ __global__ __launch_bounds__(1024) void k_scan_add(unsigned *out, const unsigned
         if (i + k < n) out[i + k] += add;
 }
 
-// one entry per non-empty particle block (keys are block * 2^DIM + octant); clears the
-// histogram for the next sort
-__global__ __launch_bounds__(256) void k_make_plist(KArgs A, unsigned *count, const unsigned *off, PBlk *plist,
-                                                    int *npblk, long long nkeys, int noct) {
+// one entry per non-empty particle block (keys are block * 2^DIM + octant), in no
+// particular order (the passes take blocks by index); k_clear then clears the histogram
+__global__ __launch_bounds__(256) void k_make_plist(KArgs A, const unsigned *off, PBlk *plist, int *npblk,
+                                                    long long nkeys, int noct) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nkeys) return;
     if (k < nkeys - 1 && k % noct == 0) {
