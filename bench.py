@@ -137,7 +137,7 @@ def run_batched(args):
     ws = W.ensemble_2d(n_sims=512, steps=args.ens_steps)
     npts = sum(w.n_points for w in ws)
     sim = GpuSim(ws, precision=MPM_PREC_FP32)
-    sim.set_option(MPM_OPT_GRAPH, 1)
+    sim.set_option(MPM_OPT_GRAPH, 0 if args.no_graph else 1)
     sim.set_option(MPM_OPT_AUTOTUNE, 1)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -234,6 +234,8 @@ def main():
     ap.add_argument("--legs", default="column,batched,served",
                     help="comma list of: column (16M column, both forms), batched (512-sim 2D ensemble), served")
     ap.add_argument("--ens-steps", type=int, default=2000)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched leg without graph replay (for kernel tracers that do not follow graphs)")
     ap.add_argument("--serve-n", type=int, default=32)
     ap.add_argument("--serve-load", type=float, default=0.75, help="arrival rate as a fraction of capacity")
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
