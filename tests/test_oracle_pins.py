@@ -418,6 +418,28 @@ def test_p2g_mass_and_momentum_conservation():
     assert np.abs(nd["f"].sum(0)).max() < 1e-11 * np.abs(nd["f"]).sum()
 
 
+def test_p2g_mass_first_moment_reproduces_positions():
+    """The map reproduces linear functions: sum_i S_ip x_i = x_p (GIMP/CPDI corners are
+    symmetric about the point; CPDI-off points of the floor layer use the linear basis at the
+    point). So after a step sum_i m_i x_i = sum_p m_p x_p for the positions the step started
+    from, on a column standing on the floor of its box, after 9 steps of settling. A node
+    index in the wrong order, a shifted element or a lost corner moves the moment by M h."""
+    w = W.column_3d(grid=(16, 16, 16), domain=(1.0, 1.0, 1.0), size=(0.5, 0.5, 0.7), steps=10)
+    ora = oracle_of(w)
+    ora.step(w.dt, 9)
+    pt = ora.points(0)
+    x0 = pt["x"]
+    ora.step(w.dt, 1)
+    nd = ora.nodes(0)
+    m = w.bodies[0].m
+    xn = _node_ijk(np.arange(ora.node_count), w.N, 3) * (1.0 / 16)
+    assert abs(nd["m"].sum() - m.sum()) <= 1e-13 * m.sum()
+    np.testing.assert_allclose(nd["m"] @ xn, m @ x0, rtol=1e-13, atol=0)
+    # the floor layer (corners x + extent/2 A below z = 0) takes the CPDI-off branch
+    ext = np.asarray(pt["extent"]).reshape(len(m), -1)[:, -1]
+    assert np.count_nonzero(x0[:, 2] < 0.5 * ext) > 20
+
+
 def test_p2g_uniform_stress_no_interior_force():
     """A uniform stress exerts no force on nodes whose support is filled symmetrically."""
     dim, L, N = 2, (1.0, 1.0), (20, 20)
