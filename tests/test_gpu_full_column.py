@@ -14,6 +14,14 @@ or a corner weight lost would move the mass by whole point masses or the moment 
 
 Tolerances: 1e-12 relative (FP64 form: rounding of the 16M-term sums); precision form 1e-6
 (FP32 weights; the point masses are the FP32 values the form stores).
+
+A rigid translation closes the whole step at the same size: a 15.86M-point block on the same
+grid, 3 or more cells from every wall, with zero gravity and zero stress, moving at a uniform
+v0. Every node it touches gets p_i / m_i = v0, no force, and the gradients of the map sum to
+zero, so after k steps x = x0 + k dt v0, v = v0 and T = 0 (the oracle holds this to 5e-14 h,
+1e-15 m/s and 3e-10 Pa on a small block). Tolerances: FP64 form 1e-9 h, 1e-12 m/s and 1e-3 Pa;
+precision form 1e-5 h (its target), 1e-6 m/s and 1 Pa (FP32 gradients sum to zero only to
+about 1e-7 / h, a strain rate of 1e-5 / s at v0 = 1 m/s).
 """
 import numpy as np
 import pytest
@@ -58,3 +66,25 @@ def test_full_column_mass_and_first_moment(prec, rel):
     err = np.abs(got - want) / np.abs(want)
     print(f"first moment rel err {err}")
     assert np.all(err <= rel), (got, want)
+
+
+@pytest.mark.parametrize("prec,tol_x,tol_v,tol_T", [(MPM_PREC_FP64, 1e-9, 1e-12, 1e-3),
+                                                   (MPM_PREC_FP32, 1e-5, 1e-6, 1.0)])
+def test_full_size_rigid_translation(prec, tol_x, tol_v, tol_T):
+    N, L = (256, 256, 128), (2.0, 2.0, 1.0)
+    h = np.array([L[d] / N[d] for d in range(3)])
+    v0 = np.array([1.0, 0.5, -0.25])
+    m, vol, x, xt = W.lattice_block(3, h, (0.5, 0.5, 3 * h[2]), (1.5, 1.5, 124 * h[2]), 2, 1500.0, velocity=v0,
+                                    jitter=0.25, seed=7)
+    body = W.Body("block", m, vol, x, xt, np.ones(len(m), np.int32), 1e7, 0.3)
+    w = W.Workload(f"translate_{len(m)}pts", 3, L, N, [body], [], 1e-5, 40, (0.0, 0.0, 0.0))
+    g = gpu_of(w, precision=prec)
+    g.step(w.dt, 40)
+    pt = g.points(0)
+    g.close()
+    assert np.all(pt["active"] != 0)
+    ex = float(np.max(np.abs(pt["x"] - (x + 40 * w.dt * v0)))) / h[0]
+    ev = float(np.max(np.abs(pt["xt"] - v0)))
+    eT = float(np.max(np.abs(pt["T6"])))
+    print(f"{len(m)} points: x {ex:.2e} h, v {ev:.2e} m/s, T {eT:.2e} Pa")
+    assert ex <= tol_x and ev <= tol_v and eT <= tol_T
