@@ -92,3 +92,23 @@ def test_body_descriptor_layout_and_boundary_check():
     keep[0][1].has_boundary = 4
     assert lib.mpm_gpu_footprint(C.byref(d), 1, G.MPM_PREC_FP64) < 0
     assert b"body 1: has_boundary must be 0..3" in lib.mpm_gpu_last_error()
+
+
+def test_create_rejects_bad_forms_before_allocating():
+    """Descriptor and form errors are reported by create before any HIP call: an unknown
+    precision form, a grid too tall for the precision form's packed cell (10 bits in z), and
+    cpdi_spec outside {0, 1} with the reference's message."""
+    lib = G.load_library()
+    w = W.two_block_3d(n_cells=16, gap_cells=1)
+    d, keep = _sim_desc(w)
+    h = C.c_void_p()
+    assert lib.mpm_gpu_create_batch(C.byref(d), 1, 7, C.byref(h)) == -1
+    assert b"unknown precision form 7" in lib.mpm_gpu_last_error()
+    tall = W.Workload("tall", 3, (1.0, 1.0, 8.0), (16, 16, 1100), w.bodies[:1], [], w.dt, 1, (0.0, 0.0, -9.81))
+    d2, keep2 = _sim_desc(tall)
+    assert lib.mpm_gpu_create_batch(C.byref(d2), 1, G.MPM_PREC_FP32, C.byref(h)) == -1
+    assert b"precision form: N[2] = 1100 exceeds 1022 cells" in lib.mpm_gpu_last_error()
+    d.grid.cpdi_spec = 3
+    assert lib.mpm_gpu_create_batch(C.byref(d), 1, G.MPM_PREC_FP64, C.byref(h)) == -1
+    assert b"Unrecognized Argument for use_cpdi: 3" in lib.mpm_gpu_last_error()
+    assert not h.value
