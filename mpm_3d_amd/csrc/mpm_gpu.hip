@@ -846,38 +846,24 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
     const long long n = h->count[body], np = h->lo.np;
     if (n > 0 && (!x || !v || !V || !m || !extent || !active))
         return fail(MPM_ERR_ARG, "x, v, V, m, extent, active are required");
-    // The sorted slots of other bodies are interleaved with this body's, so an upload
-    // brings every plane back to original order on the device (second buffer), writes
-    // this body's range there and re-sorts before the next step.
+    // The body's old points are released (unused capacity, pid -1) and the new ones are
+    // staged at slots [0, n) of the second buffer, then placed in the unused slots of the
+    // current layout in slot order; the sort before the next step puts them in block order.
+    // Other bodies hold at most their capacity, so at least B.n >= n slots are unused.
     const bool f32 = h->precision == MPM_PREC_FP32;
     const int nf = f32 ? (dim == 3 ? PW<3>::N : PW<2>::N) : (dim == 3 ? PF<3>::N : PF<2>::N);
+    const int gp = (int)std::max(1ll, ceil_div(np, 256));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    {
-        // original order in the second buffer: planes, act and pid of every point in use;
-        // unused capacity stays act 0, pid -1
-        const int gp = (int)ceil_div(np, 256);
-        HIP_TRY(hipMemsetAsync(h->act2, 0, np, h->stream));
-        hipLaunchKernelGGL(k_fill_i32, dim3(gp), dim3(256), 0, h->stream, h->pid2, 0ll, np, -1);
-        if (f32)
-            hipLaunchKernelGGL(k_unsort<uint32_t>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->Pw, np, h->Pw2, h->pid,
-                               h->pid2);
-        else
-            hipLaunchKernelGGL(k_unsort<double>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->P, np, h->P2, h->pid,
-                               h->pid2);
-        hipLaunchKernelGGL(k_unsort_act8, dim3(gp), dim3(256), 0, h->stream, np, h->act, h->act2, h->pid);
-        // this body: its points get their identity, its unused capacity none
-        if (n > 0)
-            hipLaunchKernelGGL(k_iota_range, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, h->stream, h->pid2,
-                               B.p_off, B.p_off + n);
-        if (B.n > n)
-            hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)ceil_div(B.n - n, 256)), dim3(256), 0, h->stream, h->pid2,
-                               B.p_off + n, B.p_off + B.n, -1);
+    if (B.n > 0) {
+        hipLaunchKernelGGL(k_deactivate, dim3(gp), dim3(256), 0, h->stream, np, h->act, h->pid, B.p_off,
+                           B.p_off + B.n);
         HIP_TRY(hipGetLastError());
-        if (B.n > n) HIP_TRY(hipMemsetAsync(h->act2 + B.p_off + n, 0, B.n - n, h->stream));  // unused capacity
-        if (n > 0) {
+    }
+    if (n > 0) {
+        {
             std::vector<int8_t> a8(n);
             for (long long i = 0; i < n; i++) a8[i] = active[i] ? 1 : 0;
-            HIP_TRY(hipMemcpyAsync(h->act2 + B.p_off, a8.data(), n, hipMemcpyHostToDevice, h->stream));
+            HIP_TRY(hipMemcpyAsync(h->act2, a8.data(), n, hipMemcpyHostToDevice, h->stream));
             HIP_TRY(hipStreamSynchronize(h->stream));
         }
         const int NTS = dim == 3 ? 6 : 4;
@@ -918,48 +904,52 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
                 pf[(FLNJ + 3) * n + i] = (float)extent[i];
             }
             for (int f = 0; f < nf; f++)
-                HIP_TRY(hipMemcpyAsync(h->Pw2 + (long long)f * np + B.p_off, pl.data() + (size_t)f * n,
+                HIP_TRY(hipMemcpyAsync(h->Pw2 + (long long)f * np, pl.data() + (size_t)f * n,
                                        sizeof(uint32_t) * n, hipMemcpyHostToDevice, h->stream));
             HIP_TRY(hipStreamSynchronize(h->stream));
-            swap_buffers(h);
-            h->x0[body].assign(x, x + (long long)dim * n);
-            h->uploaded[body] = 1;
-            h->grid_valid = false;
-            h->sorted = false;
-            return MPM_OK;
-        }
-        // this body's planes
-        using F3 = PF<3>;
-        using F2 = PF<2>;
-        const int FX = 0, FV = dim, FT = 2 * dim;
-        const int FVOL = dim == 3 ? F3::VOL : F2::VOL, FMASS = FVOL + 1, FEXT = FVOL + 2;
-        auto put = [&](int f, const double *src) -> hipError_t {
-            return hipMemcpyAsync(h->P2 + (long long)f * np + B.p_off, src, sizeof(double) * n, hipMemcpyHostToDevice,
-                                  h->stream);
-        };
-        for (int d = 0; d < dim; d++) {
-            HIP_TRY(put(FX + d, x + (long long)d * n));
-            HIP_TRY(put(FV + d, v + (long long)d * n));
-        }
-        std::vector<double> zeros;
-        for (int c = 0; c < NTS; c++) {
-            if (T) {
-                // 6 planes in (xx, yy, zz, xy, xz, yz); 2D keeps the first four
-                HIP_TRY(put(FT + c, T + (long long)c * n));
-            } else {
-                if (zeros.empty()) zeros.assign(n, 0.0);
-                HIP_TRY(put(FT + c, zeros.data()));
+        } else {
+            // this body's planes
+            using F3 = PF<3>;
+            using F2 = PF<2>;
+            const int FX = 0, FV = dim, FT = 2 * dim;
+            const int FVOL = dim == 3 ? F3::VOL : F2::VOL, FMASS = FVOL + 1, FEXT = FVOL + 2;
+            auto put = [&](int f, const double *src) -> hipError_t {
+                return hipMemcpyAsync(h->P2 + (long long)f * np, src, sizeof(double) * n, hipMemcpyHostToDevice,
+                                      h->stream);
+            };
+            for (int d = 0; d < dim; d++) {
+                HIP_TRY(put(FX + d, x + (long long)d * n));
+                HIP_TRY(put(FV + d, v + (long long)d * n));
             }
+            std::vector<double> zeros;
+            for (int c = 0; c < NTS; c++) {
+                if (T) {
+                    // 6 planes in (xx, yy, zz, xy, xz, yz); 2D keeps the first four
+                    HIP_TRY(put(FT + c, T + (long long)c * n));
+                } else {
+                    if (zeros.empty()) zeros.assign(n, 0.0);
+                    HIP_TRY(put(FT + c, zeros.data()));
+                }
+            }
+            HIP_TRY(put(FVOL, V));
+            HIP_TRY(put(FMASS, m));
+            HIP_TRY(put(FEXT, extent));
+            if (zeros.empty()) zeros.assign(n, 0.0);
+            HIP_TRY(put(FEXT + 1, zeros.data()));  // gammap, gammap_dot start at 0
+            HIP_TRY(put(FEXT + 2, zeros.data()));
+            HIP_TRY(hipStreamSynchronize(h->stream));
         }
-        HIP_TRY(put(FVOL, V));
-        HIP_TRY(put(FMASS, m));
-        HIP_TRY(put(FEXT, extent));
-        if (zeros.empty()) zeros.assign(n, 0.0);
-        HIP_TRY(put(FEXT + 1, zeros.data()));  // gammap, gammap_dot start at 0
-        HIP_TRY(put(FEXT + 2, zeros.data()));
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        swap_buffers(h);
+        hipLaunchKernelGGL(k_free_count, dim3(gp), dim3(256), 0, h->stream, np, h->pid, h->inact_count);
+        scan_exclusive(h, h->inact_count, h->inact_off, gp, h->scan_tmp_i);
+        if (f32)
+            hipLaunchKernelGGL(k_place<uint32_t>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->Pw2, h->Pw, h->act2,
+                               h->act, h->pid, h->inact_off, n, B.p_off);
+        else
+            hipLaunchKernelGGL(k_place<double>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->P2, h->P, h->act2,
+                               h->act, h->pid, h->inact_off, n, B.p_off);
+        HIP_TRY(hipGetLastError());
     }
+    HIP_TRY(hipStreamSynchronize(h->stream));
     h->x0[body].assign(x, x + (long long)dim * n);
     h->uploaded[body] = 1;
     h->grid_valid = false;

@@ -228,6 +228,45 @@ __global__ __launch_bounds__(256) void k_deactivate(long long np, int8_t *act, i
     }
 }
 
+// Upload into free slots. k_free_count: unused slots (pid < 0) per 256-slot block; after
+// an exclusive scan of those counts, k_place gives the j-th unused slot (in slot order)
+// staged point j < n: its planes (staged at slot j of `stage`, same plane stride), its
+// activity and original index p_off + j.
+__device__ __forceinline__ unsigned long long free_ballot(long long np, const int *pid, bool &fr,
+                                                          unsigned *wc) {
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+    fr = p < np && pid[p] < 0;
+    const unsigned long long bal = __ballot(fr);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = (unsigned)__popcll(bal);
+    __syncthreads();
+    return bal;
+}
+
+__global__ __launch_bounds__(256) void k_free_count(long long np, const int *pid, unsigned *count) {
+    __shared__ unsigned wc[4];
+    bool fr;
+    free_ballot(np, pid, fr, wc);
+    if (threadIdx.x == 0) count[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+template <class E>
+__global__ __launch_bounds__(256) void k_place(long long np, int nf, const E *stage, E *P, const int8_t *act_stage,
+                                               int8_t *act, int *pid, const unsigned *free_off, long long n,
+                                               long long p_off) {
+    __shared__ unsigned wc[4];
+    bool fr;
+    const unsigned long long bal = free_ballot(np, pid, fr, wc);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    long long r = free_off[blockIdx.x];
+    for (int i = 0; i < w; i++) r += wc[i];
+    r += __popcll(bal & ((1ull << lane) - 1ull));
+    if (!fr || r >= n) return;
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+    for (int f = 0; f < nf; f++) P[f * np + p] = stage[f * np + r];
+    act[p] = act_stage[r];
+    pid[p] = (int)(p_off + r);
+}
+
 __global__ __launch_bounds__(256) void k_unsort_act(long long np, const int8_t *act, int *dst, const int *pid) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
