@@ -180,39 +180,21 @@ __global__ __launch_bounds__(256) void k_bin_scatter(long long np, const E *P, E
     pid2[q] = o;
 }
 
-// slot order -> original order for selected planes (download, upload); slots of unused
-// capacity (pid < 0) are skipped. With pid_dst, also writes the identity pid there.
+// slot order -> original order for selected planes (downloads); slots of unused capacity
+// (pid < 0) are skipped
 template <class E>
 __global__ __launch_bounds__(256) void k_unsort(long long np, int nf, const E *src, long long src_stride, E *dst,
-                                                const int *pid, int *pid_dst = nullptr) {
+                                                const int *pid) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     const int o = pid[p];
     if (o < 0) return;
     for (int f = 0; f < nf; f++) dst[f * np + o] = src[f * src_stride + p];
-    if (pid_dst) pid_dst[o] = o;
-}
-
-__global__ __launch_bounds__(256) void k_fill_i32(int *a, long long lo, long long hi, int v) {
-    const long long p = lo + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < hi) a[p] = v;
 }
 
 __global__ __launch_bounds__(256) void k_fill_f64(double *a, long long n, double v) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
-}
-
-__global__ __launch_bounds__(256) void k_unsort_act8(long long np, const int8_t *act, int8_t *dst, const int *pid) {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    const int o = pid[p];
-    if (o >= 0) dst[o] = act[p];
-}
-
-__global__ __launch_bounds__(256) void k_iota_range(int *pid, long long lo, long long hi) {
-    const long long p = lo + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < hi) pid[p] = (int)p;
 }
 
 // release the points whose original index lies in [lo, hi): inactive and marked as
