@@ -60,24 +60,60 @@ BYTES = {
 CLASSES = {0: "pass", 1: "grid", 2: "sort", 3: "p2g", 4: "g2p"}
 
 
+def host_cpu():
+    """CPU model (/proc/cpuinfo), logical CPUs, and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = nproc
+    return model, nproc, usable
+
+
+def _time_steps(ora, dt, threads, min_s, min_steps):
+    """Steps of the oracle until at least `min_s` seconds and `min_steps` steps have passed."""
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        ora.step(dt, 1, nthreads=threads)
+        steps += 1
+        el = time.perf_counter() - t0
+        if steps >= min_steps and el >= min_s:
+            return steps, el
+
+
 def cpu_baseline(threads):
-    """Threaded oracle on the 1/16-footprint sample of the same column (same h, E, dt)."""
+    """The oracle's threaded twin of ThreadPoolExplicitUSL (persistent worker pool, -O3
+    -march=native) and its serial ExplicitUSL on the 1/16-footprint sample of the same
+    column (same h, E, dt), each timed for a few seconds on this host."""
     from mpm_3d_amd import workloads as W
     from tests._harness import oracle_of
+    model, nproc, usable = host_cpu()
     w = W.column_3d_sample()
     ora = oracle_of(w, native=True)
-    ora.step(w.dt, 1, nthreads=threads)  # warm-up (allocations)
-    steps = 3
-    t0 = time.perf_counter()
-    ora.step(w.dt, steps, nthreads=threads)
-    dt = time.perf_counter() - t0
+    ora.step(w.dt, 1, nthreads=threads)  # warm-up: allocations, worker start
+    st, el = _time_steps(ora, w.dt, threads, 4.0, 2)
+    st1, el1 = _time_steps(ora, w.dt, 1, 2.0, 1)
     return {
-        "value": w.n_points * steps / dt,
+        "value": w.n_points * st / el,
         "unit": "particle-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{w.n_points} points, 64x64x128 grid (same h, material, dt as the 16M column), "
-                  f"{steps} timed steps of the threaded ExplicitUSL restatement ({dt:.1f} s)",
+        "sample": f"{w.n_points} points, 64x64x128 grid (same h, material, dt as the 16M column): "
+                  f"{st} steps of the threaded ExplicitUSL restatement on {threads} threads in {el:.1f} s, "
+                  f"{st1} serial step(s) in {el1:.1f} s",
+        "serial_value": w.n_points * st1 / el1,
+        "host": {"cpu_model": model, "nproc": nproc, "usable_cpus": usable,
+                 "threads_used": threads},
     }
 
 
@@ -86,8 +122,10 @@ def run_form(w, form, args):
     from mpm_3d_amd.gpu import MPM_PREC_FP32, MPM_PREC_FP64, GpuSim
     prec = MPM_PREC_FP32 if form == "fp32" else MPM_PREC_FP64
     sim = GpuSim(w, precision=prec, sort_interval=args.sort_interval, fused=args.fused)
-    if args.warmup:
-        sim.step(w.dt, args.warmup)
+    # the column's first steps are faster than later ones (points spread over more dual cells
+    # as it deforms); `settle` untimed steps start the timed window in the steadier regime
+    if args.settle + args.warmup:
+        sim.step(w.dt, args.settle + args.warmup)
     sim.synchronize()
     sim.set_timing(True)
     t0 = time.perf_counter()
@@ -133,45 +171,59 @@ def run_batched(args):
     steps each; against the same simulations run one at a time (timed on a sample)."""
     from mpm_3d_amd import workloads as W
     from mpm_3d_amd.gpu import MPM_OPT_AUTOTUNE, MPM_OPT_GRAPH, MPM_OPT_GRAPH_CAPTURES, MPM_OPT_GRID_WGS, \
-        MPM_OPT_P2G_GRID, MPM_OPT_PASS_GRID, MPM_PREC_FP32, GpuSim
+        MPM_OPT_P2G_GRID, MPM_OPT_PASS_GRID, MPM_OPT_SORT_INTERVAL, MPM_PREC_FP32, GpuSim
     ws = W.ensemble_2d(n_sims=512, steps=args.ens_steps)
     npts = sum(w.n_points for w in ws)
+    # Both legs are timed the same way: handle creation and upload (setup) apart from the
+    # stepping, each reported.
+    ts = time.perf_counter()
     sim = GpuSim(ws, precision=MPM_PREC_FP32)
     sim.set_option(MPM_OPT_GRAPH, 0 if args.no_graph else 1)
     sim.set_option(MPM_OPT_AUTOTUNE, 1)
     sim.synchronize()
+    setup = time.perf_counter() - ts
     t0 = time.perf_counter()
     sim.step_batch([w.dt for w in ws], args.ens_steps)
     sim.synchronize()
     wall = time.perf_counter() - t0
     tuned = {"pass_grid": sim.get_option(MPM_OPT_PASS_GRID), "p2g_grid": sim.get_option(MPM_OPT_P2G_GRID),
-             "grid_wgs": sim.get_option(MPM_OPT_GRID_WGS),
+             "grid_wgs": sim.get_option(MPM_OPT_GRID_WGS), "sort_interval": sim.get_option(MPM_OPT_SORT_INTERVAL),
              "graph_captures": sim.get_option(MPM_OPT_GRAPH_CAPTURES)}
     sim.close()
-    print(f"[bench] batched: {len(ws)} sims, {npts} points, {args.ens_steps} steps in {wall:.2f} s", file=sys.stderr,
-          flush=True)
+    print(f"[bench] batched: {len(ws)} sims, {npts} points, {args.ens_steps} steps in {wall:.2f} s "
+          f"(setup {setup:.2f} s)", file=sys.stderr, flush=True)
     # one at a time: every 32nd simulation on its own handle (same form, same steps)
     sample = ws[::32]
-    t1 = time.perf_counter()
+    alone = alone_setup = 0.0
     for w in sample:
+        ts = time.perf_counter()
         one = GpuSim(w, precision=MPM_PREC_FP32)
+        one.synchronize()
+        t1 = time.perf_counter()
         one.step(w.dt, args.ens_steps)
         one.synchronize()
+        alone += time.perf_counter() - t1
+        alone_setup += t1 - ts
         one.close()
-    alone = time.perf_counter() - t1
     alone_pts = sum(w.n_points for w in sample)
     out = {
         "workload": f"ensemble_2d: 512 block drops, grid 16^2..256^2, {npts} points total, {args.ens_steps} steps each",
         "solves_per_s": len(ws) / wall,
         "particle_updates_per_s": npts * args.ens_steps / wall,
         "wall_s": wall,
+        "setup_s": setup,
         "tuned": tuned,
         "one_at_a_time": {"sample": len(sample), "solves_per_s": len(sample) / alone,
                           "particle_updates_per_s": alone_pts * args.ens_steps / alone,
-                          "note": "each sampled simulation created, uploaded and stepped alone"},
+                          "stepping_s": alone, "setup_s": alone_setup,
+                          "note": "each sampled simulation on its own handle; creation and upload timed "
+                                  "apart from the stepping, as in the batched leg"},
     }
     out["speedup_solves"] = out["solves_per_s"] / out["one_at_a_time"]["solves_per_s"]
     out["speedup_particle_updates"] = out["particle_updates_per_s"] / out["one_at_a_time"]["particle_updates_per_s"]
+    # the same ratio with setup counted on both sides (the sample scaled to 512 simulations)
+    scale = len(ws) / len(sample)
+    out["speedup_solves_with_setup"] = ((alone + alone_setup) * scale) / (wall + setup)
     return out
 
 
@@ -189,7 +241,7 @@ def run_served(args):
     slots, per = S.slots_for_budget(3, 128, caps, args.serve_budget_gb * 1e9)
     slots = min(slots, args.serve_n)
     pu = sum(w.n_points * st for w, st in reqs)
-    pool = S.SlotPool(slots, dim=3, N_cap=128, body_caps=caps, precision=MPM_PREC_FP32)
+    pool = S.SlotPool(slots, dim=3, N_cap=128, body_caps=caps, precision=MPM_PREC_FP32, graphs=not args.no_graph)
     sat = S.run_stream(pool, reqs, [0.0] * len(reqs), chunk_max=64)
     cap_solves = len(reqs) / sat["makespan"]
     rate = args.serve_load * cap_solves
@@ -226,6 +278,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--settle", type=int, default=200,
+                    help="untimed steps of the column before the warm-up, so the timed window is not its start")
     ap.add_argument("--precision", choices=["both", "fp32", "fp64"], default="both")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -235,7 +289,7 @@ def main():
                     help="comma list of: column (16M column, both forms), batched (512-sim 2D ensemble), served")
     ap.add_argument("--ens-steps", type=int, default=2000)
     ap.add_argument("--no-graph", action="store_true",
-                    help="batched leg without graph replay (for kernel tracers that do not follow graphs)")
+                    help="batched and served legs without graph replay (used for every profiled run)")
     ap.add_argument("--serve-n", type=int, default=32)
     ap.add_argument("--serve-load", type=float, default=0.75, help="arrival rate as a fraction of capacity")
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
@@ -274,6 +328,7 @@ def main():
         "dtype": "f32 state, f64 accumulation" if main_form == "fp32" else "f64",
         "data": "synthetic",
         "config": {"workload": w.name, "points": int(w.n_points), "grid": list(w.N), "dt": w.dt,
+                   "timed_window": f"steps {args.settle + args.warmup + 1}..{args.settle + args.warmup + args.steps}",
                    "form": "precision (FP32 state, FP64 accumulation)" if main_form == "fp32" else "plain FP64",
                    "sort_interval": args.sort_interval, "particle_pass": "fused" if args.fused else "split"},
         "roofline": r["roofline"],
@@ -290,8 +345,9 @@ def main():
     if "served" in legs:
         out["served"] = run_served(args)
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        print(f"[bench] cpu baseline with {threads} threads", file=sys.stderr, flush=True)
+        # one GPU's share of the host: at most 16 CPUs, fewer if the process may use fewer
+        threads = args.cpu_threads or min(16, host_cpu()[2])
+        print(f"[bench] cpu baseline with {threads} threads ({host_cpu()})", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(threads)
     print(json.dumps(out), flush=True)
     return 0

@@ -854,6 +854,12 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
     const int nf = f32 ? (dim == 3 ? PW<3>::N : PW<2>::N) : (dim == 3 ? PF<3>::N : PF<2>::N);
     const int gp = (int)std::max(1ll, ceil_div(np, 256));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    // From here on the body's old points are gone: until this upload completes, the body
+    // counts as not uploaded (a step reports MPM_ERR_STATE instead of running without it)
+    // and the layout as unsorted.
+    h->uploaded[body] = 0;
+    h->sorted = false;
+    h->grid_valid = false;
     if (B.n > 0) {
         hipLaunchKernelGGL(k_deactivate, dim3(gp), dim3(256), 0, h->stream, np, h->act, h->pid, B.p_off,
                            B.p_off + B.n);

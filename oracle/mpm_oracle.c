@@ -33,6 +33,7 @@
 
 #include <math.h>
 #include <pthread.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -303,19 +304,63 @@ static void generate_map_range(const ora_sim *s, ora_body *B, int64_t p0, int64_
 }
 
 /* -------------------------------------------------------------------------- */
-/* threading helper (restates include/threadpool.hpp's role: contiguous ranges)  */
+/* threading helper: a persistent worker pool, as include/threadpool.hpp:19-146 keeps one
+ * (workers started once, woken per job, the caller waiting until every job of a batch has
+ * finished, as ThreadPoolExplicitUSL waits on its jobs). A batch is one range per worker:
+ * thread t takes [n t / T, n (t + 1) / T). The pool lives until the process ends; a call
+ * with another thread count rebuilds it. */
 typedef void (*range_fn)(void *ctx, int tid, int64_t lo, int64_t hi);
-typedef struct {
+
+#define ORA_MAX_THREADS 256
+static struct {
+    int nthreads;            /* workers alive (0: none) */
+    pthread_t th[ORA_MAX_THREADS];
+    pthread_mutex_t mu;
+    pthread_cond_t wake, done;
+    unsigned long gen;       /* batch generation, bumped per parallel_for */
+    int pending;             /* workers still running the current batch */
+    int quit;
     range_fn fn;
     void *ctx;
-    int tid;
-    int64_t lo, hi;
-} range_job;
+    int64_t n;
+} g_pool = {0, {0}, PTHREAD_MUTEX_INITIALIZER, PTHREAD_COND_INITIALIZER, PTHREAD_COND_INITIALIZER, 0, 0, 0, NULL,
+            NULL, 0};
 
-static void *range_entry(void *arg) {
-    range_job *j = (range_job *)arg;
-    j->fn(j->ctx, j->tid, j->lo, j->hi);
+static void *pool_worker(void *arg) {
+    const int tid = (int)(intptr_t)arg;
+    unsigned long seen = 0;
+    pthread_mutex_lock(&g_pool.mu);
+    for (;;) {
+        while (!g_pool.quit && g_pool.gen == seen) pthread_cond_wait(&g_pool.wake, &g_pool.mu);
+        if (g_pool.quit) break;
+        seen = g_pool.gen;
+        const range_fn fn = g_pool.fn;
+        void *ctx = g_pool.ctx;
+        const int64_t n = g_pool.n;
+        const int T = g_pool.nthreads;
+        pthread_mutex_unlock(&g_pool.mu);
+        fn(ctx, tid, n * tid / T, n * (tid + 1) / T);
+        pthread_mutex_lock(&g_pool.mu);
+        if (--g_pool.pending == 0) pthread_cond_signal(&g_pool.done);
+    }
+    pthread_mutex_unlock(&g_pool.mu);
     return NULL;
+}
+
+static void pool_resize(int nthreads) {
+    if (g_pool.nthreads == nthreads) return;
+    if (g_pool.nthreads > 0) {
+        pthread_mutex_lock(&g_pool.mu);
+        g_pool.quit = 1;
+        pthread_cond_broadcast(&g_pool.wake);
+        pthread_mutex_unlock(&g_pool.mu);
+        for (int t = 0; t < g_pool.nthreads; t++) pthread_join(g_pool.th[t], NULL);
+        g_pool.quit = 0;
+        g_pool.nthreads = 0;
+    }
+    g_pool.gen = 0;
+    g_pool.nthreads = nthreads;
+    for (int t = 0; t < nthreads; t++) pthread_create(&g_pool.th[t], NULL, pool_worker, (void *)(intptr_t)t);
 }
 
 /* every thread id 0..nthreads-1 runs exactly once (ranges may be empty) */
@@ -324,18 +369,17 @@ static void parallel_for(int nthreads, int64_t n, range_fn fn, void *ctx) {
         fn(ctx, 0, 0, n);
         return;
     }
-    pthread_t th[256];
-    range_job jobs[256];
-    if (nthreads > 256) nthreads = 256;
-    for (int t = 0; t < nthreads; t++) {
-        jobs[t].fn = fn;
-        jobs[t].ctx = ctx;
-        jobs[t].tid = t;
-        jobs[t].lo = n * t / nthreads;
-        jobs[t].hi = n * (t + 1) / nthreads;
-        pthread_create(&th[t], NULL, range_entry, &jobs[t]);
-    }
-    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    if (nthreads > ORA_MAX_THREADS) nthreads = ORA_MAX_THREADS;
+    pool_resize(nthreads);
+    pthread_mutex_lock(&g_pool.mu);
+    g_pool.fn = fn;
+    g_pool.ctx = ctx;
+    g_pool.n = n;
+    g_pool.pending = nthreads;
+    g_pool.gen++;
+    pthread_cond_broadcast(&g_pool.wake);
+    while (g_pool.pending > 0) pthread_cond_wait(&g_pool.done, &g_pool.mu);
+    pthread_mutex_unlock(&g_pool.mu);
 }
 
 /* ThreadPoolPoints::generateMap (threadpool_points.cpp:202-349): per-thread COO over

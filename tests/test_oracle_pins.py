@@ -540,6 +540,58 @@ def test_contact_friction_limits():
             assert np.abs(((v1 - v2) * n).sum(1)).max() < 1e-12 * vscale
 
 
+@pytest.mark.parametrize("contact_spec", [0, 1])
+def test_contact_sticking_explicit_vs_implicit(contact_spec):
+    """mu -> infinity, g = 0, and a different prestress in each body, so that the bodies'
+    nodal forces before contact differ (f1/m1 != f2/m2 at the engaged nodes). From
+    contact_huang.cpp:131-184 with the full tangential force, moveGrid gives:
+    IMPLICIT (mv_k = p_k + dt f_k): v1 = v2 = (p1 + p2 + dt (f1 + f2)) / (m1 + m2);
+    EXPLICIT (mv_k = p_k): v_k = vcm + dt f_k / m_k with vcm = (p1 + p2) / (m1 + m2), so
+    v1 - v2 = dt (f1/m1 - f2/m2). f_k and p_k are the nodal force and momentum of the same
+    step without the contact. An EXPLICIT branch that used p + dt f, or swapped branches,
+    fails one of the two."""
+    dt = 1e-4
+
+    def build(with_contact):
+        L, N = (1.0, 1.0), (32, 32)
+        h = 1.0 / 32
+        x1 = 0.5 - h / 2
+        b1 = _block(2, L, N, (x1 - 0.25, 0.3), (x1, 0.55), velocity=(1.0, 0.2), jitter=0.2, seed=1)
+        b2 = _block(2, L, N, (1 - x1, 0.35), (1.25 - x1, 0.6), velocity=(-1.0, -0.1), jitter=0.2, seed=2)
+        contacts = [W.Contact(0, 1, 1e12)] if with_contact else []
+        ora, w = _sim(2, L, N, [b1, b2], contacts=contacts, contact_spec=contact_spec)
+        ora.set_points(0, T=np.tile(np.array([[4e3, 1e3, 0], [1e3, -2e3, 0], [0, 0, 0]]).reshape(1, 9), (b1.n, 1)))
+        ora.set_points(1, T=np.tile(np.array([[-3e3, 0, 0], [0, 5e3, 0], [0, 0, 0]]).reshape(1, 9), (b2.n, 1)))
+        return ora
+
+    free = build(False)
+    free.step(dt, 1)
+    n1, n2 = free.nodes(0), free.nodes(1)
+    ora = build(True)
+    ora.step(dt, 1)
+    _, frc = ora.contact(0)
+    eng = np.any(frc != 0, axis=1)
+    assert eng.sum() > 5
+    m1, m2 = n1["m"][eng, None], n2["m"][eng, None]
+    f1, f2 = n1["f"][eng, :2], n2["f"][eng, :2]
+    # momentum before the grid update: p after moveGrid minus dt f (no boundary acts there)
+    p1 = n1["mxt"][eng, :2] - dt * f1
+    p2 = n2["mxt"][eng, :2] - dt * f2
+    dacc = dt * (f1 / m1 - f2 / m2)
+    assert np.abs(dacc).max() > 1e-6  # the prestresses make the bodies' accelerations differ
+    v1 = ora.nodes(0)["xt"][eng, :2]
+    v2 = ora.nodes(1)["xt"][eng, :2]
+    vscale = 2.0
+    if contact_spec == 0:
+        vcm = (p1 + p2) / (m1 + m2)
+        np.testing.assert_allclose(v1 - v2, dacc, rtol=0, atol=1e-10 * vscale)
+        np.testing.assert_allclose(v1, vcm + dt * f1 / m1, rtol=0, atol=1e-10 * vscale)
+    else:
+        vcm = (p1 + p2 + dt * (f1 + f2)) / (m1 + m2)
+        assert np.abs(v1 - v2).max() < 1e-12 * vscale
+        np.testing.assert_allclose(v1, vcm, rtol=0, atol=1e-10 * vscale)
+
+
 def test_threaded_twin_matches_serial():
     """The threaded restatement differs only in summation order."""
     w = example_workload("collision2D")
