@@ -16,6 +16,8 @@ The JSON line also carries
                 time (HIP events on the library's stream), against 8 TB/s
   cpu_baseline  the oracle's threaded twin of ThreadPoolExplicitUSL (oracle/, -O3
                 -march=native) on a bounded sample of the same workload, on this host
+  collision     the two-block collision on 256^3 (7.3M points, two bodies, ContactHuang),
+                plain FP64 form, as configured and with contact engaged
   batched       512 independent 2D block drops (16^2..256^2 grids) in one handle, precision
                 form, per-simulation dt, HIP-graph sort periods, autotuned launch shapes:
                 solves/s and particle-updates/s against the same simulations one at a time
@@ -117,15 +119,16 @@ def cpu_baseline(threads):
     }
 
 
-def run_form(w, form, args):
+def run_form(w, form, args, settle=None, traffic_key="column_3d_16M"):
     """Time `args.steps` steps of one form; returns its summary."""
     from mpm_3d_amd.gpu import MPM_PREC_FP32, MPM_PREC_FP64, GpuSim
     prec = MPM_PREC_FP32 if form == "fp32" else MPM_PREC_FP64
     sim = GpuSim(w, precision=prec, sort_interval=args.sort_interval, fused=args.fused)
     # the column's first steps are faster than later ones (points spread over more dual cells
     # as it deforms); `settle` untimed steps start the timed window in the steadier regime
-    if args.settle + args.warmup:
-        sim.step(w.dt, args.settle + args.warmup)
+    settle = args.settle if settle is None else settle
+    if settle + args.warmup:
+        sim.step(w.dt, settle + args.warmup)
     sim.synchronize()
     sim.set_timing(True)
     t0 = time.perf_counter()
@@ -147,7 +150,7 @@ def run_form(w, form, args):
     bytes_per_launch = bp * active_points + bn * active_nodes
     achieved = bytes_per_launch / (kern[dom]["avg_ms"] * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{form}_column_3d_16M.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{form}_{traffic_key}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -163,6 +166,26 @@ def run_form(w, form, args):
         "step_hbm_fraction": round(step_bytes / wall / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": kern,
     }
+
+
+def run_collision(args):
+    """Two-block collision on 256^3 (two 0.3 m cubes at 2^3 points per cell, 7.2M points,
+    +-1 m/s, E = 1e6, ContactHuang mu = 0.4, CartesianBox walls, dt = 2e-5) in the plain
+    FP64 form, as configured (centres 0.4 m apart: the cubes do not meet within its 1000
+    steps) and with the faces one cell apart so that contact is engaged from the first
+    steps; `args.steps` timed steps after `args.warmup`."""
+    from mpm_3d_amd import workloads as W
+    out = {}
+    for name, gap in (("as_configured", None), ("contact_engaged", 1)):
+        w = W.two_block_3d(n_cells=256, gap_cells=gap)
+        r = run_form(w, "fp64", args, settle=0, traffic_key="collision_3d_256")
+        out[name] = {"workload": f"{w.name}: {w.n_points} points, centres "
+                                 f"{'0.4 m apart' if gap is None else 'one cell apart at the faces'}",
+                     "value": r["value"], "ms_per_step": r["ms_per_step"], "roofline": r["roofline"],
+                     "kernels": r["kernels"]}
+        print(f"[bench] collision {name}: {r['value']:.3e} particle-updates/s, {r['ms_per_step']:.3f} ms/step",
+              file=sys.stderr, flush=True)
+    return out
 
 
 def run_batched(args):
@@ -285,8 +308,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--sort-interval", type=int, default=16)
     ap.add_argument("--fused", action="store_true", help="one fused G2P+P2G pass per step instead of two passes")
-    ap.add_argument("--legs", default="column,batched,served",
-                    help="comma list of: column (16M column, both forms), batched (512-sim 2D ensemble), served")
+    ap.add_argument("--legs", default="column,collision,batched,served",
+                    help="comma list of: column (16M column, both forms), collision (256^3 two-block "
+                         "collision, FP64), batched (512-sim 2D ensemble), served")
     ap.add_argument("--ens-steps", type=int, default=2000)
     ap.add_argument("--no-graph", action="store_true",
                     help="batched and served legs without graph replay (used for every profiled run)")
@@ -340,6 +364,8 @@ def main():
     if len(res) == 2:
         out["fp32_vs_fp64"] = round(res["fp32"]["value"] / res["fp64"]["value"], 3)
     legs = set(args.legs.split(","))
+    if "collision" in legs:
+        out["collision"] = run_collision(args)
     if "batched" in legs:
         out["batched"] = run_batched(args)
     if "served" in legs:
