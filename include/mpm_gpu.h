@@ -180,6 +180,18 @@ int mpm_gpu_download_contact(mpm_gpu_handle *h, int contact, double *normal, dou
  * (default_points.cpp:196-330, cartesian.cpp:219-253). */
 int mpm_gpu_connectivity(mpm_gpu_handle *h, int body, int32_t *corner_elem, uint8_t *code);
 
+/* No counterpart in the reference. The same result as mpm_gpu_connectivity, for callers that
+ * cannot allocate device memory (code between HIP graph captures, serving loops that must not
+ * call hipMalloc): it calls no hipMalloc/hipFree or allocating library routine. Its kernel
+ * writes into a fixed device array of the library (MPM_NOALLOC_MAX_POINTS points x 2^dim
+ * corner elements + codes, 16.5 MiB held on each device the library runs on until the process
+ * ends) instead of a per-call buffer. Bodies with more than MPM_NOALLOC_MAX_POINTS points are
+ * refused with MPM_ERR_ARG. Every call on a device shares that device's array: calls that
+ * overlap on one device (other threads, other handles) race and must be serialised by the
+ * caller. The handle's device (current at create) must be current (else MPM_ERR_STATE). */
+#define MPM_NOALLOC_MAX_POINTS 524288
+int mpm_gpu_connectivity_noalloc(mpm_gpu_handle *h, int body, int32_t *corner_elem, uint8_t *code);
+
 /* Points re-sorted by grid block every `interval` steps, counted across mpm_gpu_step
  * calls (default 16; 0 = at the start of every mpm_gpu_step call). An upload always
  * forces a sort before the next step. Affects speed only. */

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <string>
 #include <limits>
+#include <mutex>
 #include <vector>
 
 #include "mpm_gpu.h"
@@ -215,6 +216,7 @@ struct Timer {
 struct mpm_gpu_handle {
     Layout lo;
     int precision = 0;
+    int device = 0;  // the device current at create
     hipStream_t stream = nullptr;
     SimDev *d_sims = nullptr;
     BodyDev *d_bodies = nullptr;
@@ -691,7 +693,9 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
         rc = (x);                \
         if (rc) return cleanup(rc); \
     } while (0)
-    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    hipError_t e = hipGetDevice(&h->device);
+    if (e != hipSuccess) return cleanup(fail(MPM_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e)));
+    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return cleanup(fail(MPM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)));
     const Layout &L = h->lo;
     const int dim = L.dim;
@@ -1358,6 +1362,51 @@ static long long sim_nodes(const SimDev &s, int dim) {
     return nn;
 }
 
+// k_connectivity over the points with original index in [o_lo, o_lo + o_n), into elem
+// (o_n x 2^dim) and code (o_n)
+static hipError_t launch_connectivity(mpm_gpu_handle *h, int *elem, uint8_t *code, long long o_lo, long long o_n) {
+    KArgs A = kargs(h);
+    const unsigned gp = (unsigned)ceil_div(h->lo.np, 256);
+    const bool f32 = h->precision == MPM_PREC_FP32;
+    if (h->lo.dim == 3) {
+        if (f32) hipLaunchKernelGGL((k_connectivity<3, Prec32>), dim3(gp), dim3(256), 0, h->stream, A, elem, code, o_lo, o_n);
+        else hipLaunchKernelGGL((k_connectivity<3, Prec64>), dim3(gp), dim3(256), 0, h->stream, A, elem, code, o_lo, o_n);
+    } else {
+        if (f32) hipLaunchKernelGGL((k_connectivity<2, Prec32>), dim3(gp), dim3(256), 0, h->stream, A, elem, code, o_lo, o_n);
+        else hipLaunchKernelGGL((k_connectivity<2, Prec64>), dim3(gp), dim3(256), 0, h->stream, A, elem, code, o_lo, o_n);
+    }
+    return hipGetLastError();
+}
+
+// Device scratch of mpm_gpu_connectivity_noalloc: corner elements (MPM_NOALLOC_MAX_POINTS x 8
+// int32) then codes (MPM_NOALLOC_MAX_POINTS bytes), 16.5 MiB held on every device the library
+// runs on until the process ends. In host code `scratch` names a host-side stand-in, not
+// device memory: its device address comes from hipGetSymbolAddress, once per device.
+constexpr size_t SCRATCH_CODE_OFF = (size_t)MPM_NOALLOC_MAX_POINTS * 8 * sizeof(int);
+static __device__ __align__(256) unsigned char scratch[SCRATCH_CODE_OFF + (size_t)MPM_NOALLOC_MAX_POINTS];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// The scratch's device address on the handle's device (looked up on that device's first call)
+static int scratch_of(mpm_gpu_handle *h, unsigned char **out) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != h->device)
+        return fail(MPM_ERR_STATE, "the handle was created on device %d; device %d is current", h->device, dev);
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return fail(MPM_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(MPM_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return MPM_OK;
+}
+
 template <int DIM, class GO>
 static void launch_nodes_out(mpm_gpu_handle *h, int si, long long grid_base, const int *lst, int cnt, int with_rec,
                              int contact, double *out, long long nn) {
@@ -1455,22 +1504,36 @@ int mpm_gpu_connectivity(mpm_gpu_handle *h, int body, int32_t *corner_elem, uint
     uint8_t *d_code = nullptr;
     HIP_TRY(hipMalloc(&d_elem, sizeof(int) * np * nc));
     HIP_TRY(hipMalloc(&d_code, np));
-    KArgs A = kargs(h);
-    const int gp = (int)ceil_div(np, 256);
-    const bool f32 = h->precision == MPM_PREC_FP32;
-    if (dim == 3) {
-        if (f32) hipLaunchKernelGGL((k_connectivity<3, Prec32>), dim3(gp), dim3(256), 0, h->stream, A, d_elem, d_code);
-        else hipLaunchKernelGGL((k_connectivity<3, Prec64>), dim3(gp), dim3(256), 0, h->stream, A, d_elem, d_code);
-    } else {
-        if (f32) hipLaunchKernelGGL((k_connectivity<2, Prec32>), dim3(gp), dim3(256), 0, h->stream, A, d_elem, d_code);
-        else hipLaunchKernelGGL((k_connectivity<2, Prec64>), dim3(gp), dim3(256), 0, h->stream, A, d_elem, d_code);
-    }
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_connectivity(h, d_elem, d_code, 0, np));
     HIP_TRY(hipMemcpyAsync(corner_elem, d_elem + B.p_off * nc, sizeof(int) * n * nc, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipMemcpyAsync(code, d_code + B.p_off, n, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipFree(d_elem));
     HIP_TRY(hipFree(d_code));
+    return MPM_OK;
+}
+
+int mpm_gpu_connectivity_noalloc(mpm_gpu_handle *h, int body, int32_t *corner_elem, uint8_t *code) {
+    if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
+    if (body < 0 || body >= (int)h->lo.bodies.size()) return fail(MPM_ERR_ARG, "body %d out of range", body);
+    if (!corner_elem || !code) return fail(MPM_ERR_ARG, "corner_elem and code are required");
+    const BodyDev &B = h->lo.bodies[body];
+    const int nc = 1 << h->lo.dim;
+    const long long n = h->count[body];
+    if (n > MPM_NOALLOC_MAX_POINTS)
+        return fail(MPM_ERR_ARG, "body %d has %lld points; the allocation-free connectivity takes at most %lld", body,
+                    n, (long long)MPM_NOALLOC_MAX_POINTS);
+    if (n == 0) return MPM_OK;
+    unsigned char *scr = nullptr;
+    int rc = scratch_of(h, &scr);
+    if (rc) return rc;
+    // the body's points only: corner elements of point o at scr[o * nc], codes after them
+    int *d_elem = reinterpret_cast<int *>(scr);
+    uint8_t *d_code = scr + SCRATCH_CODE_OFF;
+    HIP_TRY(launch_connectivity(h, d_elem, d_code, B.p_off, n));
+    HIP_TRY(hipMemcpyAsync(corner_elem, d_elem, sizeof(int) * n * nc, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(code, d_code, n, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
     return MPM_OK;
 }
 

@@ -1197,20 +1197,23 @@ __global__ __launch_bounds__(256) void k_nodes_out(KArgs A, int si, long long gr
 }
 
 // ---------------------------------------------------------------------------------------
-// connectivity of the current state (DefaultPoints::generateMap's element lookups)
+// connectivity of the current state (DefaultPoints::generateMap's element lookups), for the
+// points whose original index lies in [o_lo, o_lo + o_n); point o goes to entry o - o_lo
 template <int DIM, class PT>
-__global__ void k_connectivity(KArgs A, int *elem_out, uint8_t *code_out) {
+__global__ void k_connectivity(KArgs A, int *elem_out, uint8_t *code_out, long long o_lo, long long o_n) {
     constexpr int NC = 1 << DIM;
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= A.np) return;
-    const int o = A.pid[p];
-    if (o < 0) return;
+    const int oa = A.pid[p];
+    if (oa < 0) return;
+    const long long o = (long long)oa - o_lo;
+    if (o < 0 || o >= o_n) return;
     int bi = 0;
     {
         int lo = 0, hi = A.nbody - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (A.bodies[mid].p_off <= o) lo = mid;
+            if (A.bodies[mid].p_off <= oa) lo = mid;
             else hi = mid - 1;
         }
         bi = lo;

@@ -24,8 +24,8 @@ MPM_MAX_BODIES_PER_SIM = 4
 
 SYMBOLS = (
     "mpm_gpu_create", "mpm_gpu_create_batch", "mpm_gpu_upload", "mpm_gpu_step", "mpm_gpu_download",
-    "mpm_gpu_download_aux", "mpm_gpu_connectivity", "mpm_gpu_set_sort_interval", "mpm_gpu_set_timing",
-    "mpm_gpu_get_timing", "mpm_gpu_stats", "mpm_gpu_synchronize", "mpm_gpu_destroy",
+    "mpm_gpu_download_aux", "mpm_gpu_connectivity", "mpm_gpu_connectivity_noalloc", "mpm_gpu_set_sort_interval",
+    "mpm_gpu_set_timing", "mpm_gpu_get_timing", "mpm_gpu_stats", "mpm_gpu_synchronize", "mpm_gpu_destroy",
     "mpm_gpu_last_error", "mpm_gpu_footprint", "mpm_gpu_host_weights", "mpm_gpu_copy_probe",
     "mpm_gpu_set_option", "mpm_gpu_get_option", "mpm_gpu_step_batch", "mpm_gpu_load_sim", "mpm_gpu_clear_sim",
     "mpm_gpu_download_nodes", "mpm_gpu_download_contact", "mpm_gpu_download_plastic",
@@ -89,6 +89,7 @@ def load_library(path: str = LIB_PATH):
     lib.mpm_gpu_download_contact.argtypes = [vp, C.c_int, _dp, _dp]
     lib.mpm_gpu_download_plastic.argtypes = [vp, C.c_int, _dp, _dp]
     lib.mpm_gpu_connectivity.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)]
+    lib.mpm_gpu_connectivity_noalloc.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)]
     lib.mpm_gpu_set_sort_interval.argtypes = [vp, C.c_int]
     lib.mpm_gpu_set_timing.argtypes = [vp, C.c_int]
     lib.mpm_gpu_set_option.argtypes = [vp, C.c_int, C.c_int]
@@ -305,13 +306,15 @@ class GpuSim:
         _check(self.lib.mpm_gpu_download_contact(self.h, c, n.ctypes.data_as(_dp), f.ctypes.data_as(_dp)))
         return n.T.copy(), f.T.copy()
 
-    def connectivity(self, gb):
+    def connectivity(self, gb, noalloc=False):
+        """Corner elements (n, 2^dim) and codes (n,) of global body gb, original order; with
+        `noalloc` through mpm_gpu_connectivity_noalloc (no device allocation)."""
         n = self._body(gb).n
         nc = 2 ** self.dim
         elem = np.zeros(n * nc, np.int32)
         code = np.zeros(n, np.uint8)
-        _check(self.lib.mpm_gpu_connectivity(self.h, gb, elem.ctypes.data_as(C.POINTER(C.c_int32)),
-                                             code.ctypes.data_as(C.POINTER(C.c_uint8))))
+        fn = self.lib.mpm_gpu_connectivity_noalloc if noalloc else self.lib.mpm_gpu_connectivity
+        _check(fn(self.h, gb, elem.ctypes.data_as(C.POINTER(C.c_int32)), code.ctypes.data_as(C.POINTER(C.c_uint8))))
         return elem.reshape(n, nc), code
 
     # -- instrumentation --------------------------------------------------------------
