@@ -21,10 +21,11 @@ The JSON line also carries
   batched       512 independent 2D block drops (16^2..256^2 grids) in one handle, precision
                 form, per-simulation dt, HIP-graph sort periods, autotuned launch shapes:
                 solves/s and particle-updates/s against the same simulations one at a time
-  served        3D block drops / collisions admitted into a slot pool sized by device
-                footprint, continuous batching: capacity (all queued at once) against the
-                same requests one at a time, and p50/p99 latency under Poisson arrivals
-                at 75% of that capacity
+  served        128 3D block drops / collisions admitted into 36 slots of three capacity
+                classes (each request into the smallest free slot that fits), continuous
+                batching: capacity (all queued at once) against the same requests one at a
+                time (sampled), p50/p99 latency under Poisson arrivals at 75% of that
+                capacity, and the mix's bound on the speed-up
 """
 from __future__ import annotations
 
@@ -250,49 +251,70 @@ def run_batched(args):
     return out
 
 
-def run_served(args):
+# Slot classes of the served leg: the largest request each N range of the served-stream
+# generator draws (drops of up to min(N/2 - 2, 62) cells a side, collisions of two cubes of up
+# to min((N - 6)/2 - 1, 49) cells, 8 points per cell), body 0 holding either, body 1 a cube
+SERVE_CLASSES = ((64, (216_000, 175_616)), (96, (778_688, 681_472)), (128, (1_906_624, 941_192)))
+
+
+def run_served(args, full_rate=None):
     """Served 3D stream: `serve_n` requests (seeded block drops / two-block collisions,
-    N^3 grids 32..128, 1e4..2e6 points, 500..5000 steps) through a slot pool sized by
-    device footprint with continuous batching. Capacity: all requests queued at t = 0,
-    against the same requests run one at a time. Latency: Poisson arrivals at
-    `serve_load` x the measured capacity (solves/s); latency = completion - arrival."""
+    N^3 grids 32..128, 1e4..2e6 points, 500..5000 steps) through a pool of slot classes
+    (N_cap 64, 96 and 128, `serve_slots` slots each, fewer slots than requests) with
+    continuous batching: each request takes the smallest free slot that fits it, and waits
+    when none does. Capacity: all requests queued at t = 0. Latency: Poisson arrivals at
+    `serve_load` x the measured capacity (solves/s); latency = completion - arrival, the
+    result downloaded. One at a time: every `serve_sample`-th request on a handle of its
+    own (create, upload, step, download), its time scaled to the whole stream. The bound:
+    one-at-a-time time over the time the stream's particle-updates take at `full_rate`
+    (the 16M column's rate, the GPU full), which no schedule can beat."""
     from mpm_3d_amd import serve as S
     from mpm_3d_amd import workloads as W
     from mpm_3d_amd.gpu import MPM_PREC_FP32
     reqs = W.serve_requests(args.serve_n)
-    caps = (2_000_000, 1_000_000)
-    slots, per = S.slots_for_budget(3, 128, caps, args.serve_budget_gb * 1e9)
-    slots = min(slots, args.serve_n)
+    counts = [int(c) for c in args.serve_slots.split(",")]
+    classes = [S.SlotClass(c, N, caps) for c, (N, caps) in zip(counts, SERVE_CLASSES)]
     pu = sum(w.n_points * st for w, st in reqs)
-    pool = S.SlotPool(slots, dim=3, N_cap=128, body_caps=caps, precision=MPM_PREC_FP32, graphs=not args.no_graph)
+    pool = S.SlotPool(classes=classes, dim=3, precision=MPM_PREC_FP32, graphs=not args.no_graph)
     sat = S.run_stream(pool, reqs, [0.0] * len(reqs), chunk_max=64)
     cap_solves = len(reqs) / sat["makespan"]
     rate = args.serve_load * cap_solves
     arrivals = S.poisson_arrivals(len(reqs), rate)
     res = S.run_stream(pool, reqs, arrivals, chunk_max=64)
+    footprint = sum(pool.footprint)
     pool.close()
     lat = np.array([f - a for f, a in zip(res["finish"], res["arrival"])])
-    print(f"[bench] served: {len(reqs)} requests on {slots} slots; capacity run {sat['makespan']:.2f} s; "
-          f"at {rate:.2f}/s p50 {np.percentile(lat, 50):.3f} s p99 {np.percentile(lat, 99):.3f} s", file=sys.stderr,
-          flush=True)
+    print(f"[bench] served: {len(reqs)} requests on {pool.n_slots} slots ({footprint / 1e9:.1f} GB); capacity run "
+          f"{sat['makespan']:.2f} s; at {rate:.2f}/s p50 {np.percentile(lat, 50):.3f} s p99 "
+          f"{np.percentile(lat, 99):.3f} s", file=sys.stderr, flush=True)
+    sample = list(range(0, len(reqs), args.serve_sample))
     t1 = time.perf_counter()
-    for w, st in reqs:
-        S.run_alone(w, st)
-    alone = time.perf_counter() - t1
-    print(f"[bench] served: one at a time {alone:.2f} s", file=sys.stderr, flush=True)
+    for i in sample:
+        S.run_alone(*reqs[i])
+    alone_sample = time.perf_counter() - t1
+    alone = alone_sample * len(reqs) / len(sample)
+    print(f"[bench] served: one at a time {alone_sample:.2f} s for {len(sample)} requests", file=sys.stderr,
+          flush=True)
     out = {
         "workload": f"{len(reqs)} requests (block drop / two-block collision, N^3 = 32^3..128^3, "
                     f"{min(w.n_points for w, _ in reqs)}..{max(w.n_points for w, _ in reqs)} points, "
                     f"500..5000 steps, {pu:.3e} particle-updates in all)",
-        "slots": slots, "slot_footprint_bytes": per,
+        "slots": {"classes": [{"count": c.count, "N_cap": c.N_cap, "body_caps": list(c.body_caps)} for c in classes],
+                  "total": pool.n_slots, "footprint_bytes": footprint},
+        "statuses": {st: sat["status"].count(st) for st in set(sat["status"])},
         "capacity": {"solves_per_s": cap_solves, "particle_updates_per_s": pu / sat["makespan"],
                      "makespan_s": sat["makespan"], "stepping_s": sat["step_s"], "admission_s": sat["admin_s"]},
         "latency": {"arrival_rate_per_s": rate, "load": args.serve_load,
                     "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
                     "solves_per_s": len(reqs) / res["makespan"], "makespan_s": res["makespan"]},
-        "one_at_a_time": {"solves_per_s": len(reqs) / alone, "particle_updates_per_s": pu / alone, "wall_s": alone},
+        "one_at_a_time": {"solves_per_s": len(reqs) / alone, "particle_updates_per_s": pu / alone,
+                          "wall_s_estimated": alone, "sample": len(sample), "sample_wall_s": alone_sample},
     }
     out["speedup_solves"] = cap_solves / out["one_at_a_time"]["solves_per_s"]
+    if full_rate:
+        t_min = pu / full_rate
+        out["bound"] = {"full_gpu_particle_updates_per_s": full_rate, "min_makespan_s": t_min,
+                        "max_speedup": alone / t_min}
     return out
 
 
@@ -314,9 +336,12 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=2000)
     ap.add_argument("--no-graph", action="store_true",
                     help="batched and served legs without graph replay (used for every profiled run)")
-    ap.add_argument("--serve-n", type=int, default=32)
+    ap.add_argument("--serve-n", type=int, default=128)
+    ap.add_argument("--serve-slots", default="16,12,8",
+                    help="slots of the N_cap 64, 96 and 128 classes of the served pool")
+    ap.add_argument("--serve-sample", type=int, default=4,
+                    help="time every n-th served request alone for the one-at-a-time figure")
     ap.add_argument("--serve-load", type=float, default=0.75, help="arrival rate as a fraction of capacity")
-    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 10 launches) used to calibrate counters")
     args = ap.parse_args()
@@ -369,7 +394,7 @@ def main():
     if "batched" in legs:
         out["batched"] = run_batched(args)
     if "served" in legs:
-        out["served"] = run_served(args)
+        out["served"] = run_served(args, res["fp32"]["value"] if "fp32" in res else None)
     if not args.no_cpu:
         # one GPU's share of the host: at most 16 CPUs, fewer if the process may use fewer
         threads = args.cpu_threads or min(16, host_cpu()[2])

@@ -206,8 +206,10 @@ int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
 /*   MPM_OPT_GRAPH          1: replay each sort period (a sort + sort_interval steps) as
  *                          a captured HIP graph (split passes, timing off); 0 (default)
  *   MPM_OPT_AUTOTUNE       1: the next step call first times candidate launch shapes
- *                          (persistent particle-pass grid, grid-update grid) on its own
- *                          first steps and keeps the fastest; those steps count
+ *                          (persistent particle-pass grids, grid-update grid) and then, when
+ *                          it has 57 steps to spare, sort intervals 8, 16 and 32, on its own
+ *                          first steps, and keeps the fastest; those steps count. Reads 2
+ *                          once tuned
  *   MPM_OPT_PASS_GRID      workgroups of the persistent G2P and fused particle passes
  *                          (default 1024); setting it sets MPM_OPT_P2G_GRID too
  *   MPM_OPT_P2G_GRID       workgroups of the persistent P2G pass (default: as many as the
@@ -219,7 +221,12 @@ int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
  *                          and the contacts' normals and forces, for mpm_gpu_download_nodes
  *                          and mpm_gpu_download_contact ((1 + 2 dim) doubles per node plus
  *                          2 dim per node of each contact's sim, allocated on first use and
- *                          not counted by mpm_gpu_footprint); graph replay is off meanwhile */
+ *                          not counted by mpm_gpu_footprint); graph replay is off meanwhile
+ *   MPM_OPT_KEEP_L         1 (default): the last step of every step call writes the velocity
+ *                          gradient for mpm_gpu_download_aux (9 doubles per point slot,
+ *                          allocated at the first step and not counted by mpm_gpu_footprint);
+ *                          0: not kept, the plane is freed and mpm_gpu_download_aux's L
+ *                          returns MPM_ERR_STATE */
 #define MPM_OPT_GRAPH 3
 #define MPM_OPT_AUTOTUNE 4
 #define MPM_OPT_PASS_GRID 5
@@ -227,6 +234,7 @@ int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
 #define MPM_OPT_GRAPH_CAPTURES 7
 #define MPM_OPT_P2G_GRID 8
 #define MPM_OPT_NODE_RECORD 9
+#define MPM_OPT_KEEP_L 10
 int mpm_gpu_set_option(mpm_gpu_handle *h, int key, int value);
 int mpm_gpu_get_option(mpm_gpu_handle *h, int key, int *value);
 
@@ -256,7 +264,9 @@ int mpm_gpu_copy_probe(int64_t nbytes, int reps, double *gbps);
 int mpm_gpu_synchronize(mpm_gpu_handle *h);
 int mpm_gpu_destroy(mpm_gpu_handle *h);
 const char *mpm_gpu_last_error(void);
-/* bytes of device memory a handle with these sims would allocate */
+/* bytes of device memory a handle with these sims allocates at create (the velocity gradient
+ * of MPM_OPT_KEEP_L and the node record of MPM_OPT_NODE_RECORD are allocated on first use
+ * and not counted) */
 int64_t mpm_gpu_footprint(const mpm_sim_desc *sims, int nsim, int precision);
 
 #ifdef __cplusplus

@@ -197,7 +197,6 @@ long long footprint_bytes(const Layout &Lo, int precision) {
     long long b = 0;
     b += 2 * Lo.np * (nf * esz + 1 + 4);   // particle planes, double-buffered, + act + pid
     b += Lo.np * (4 + 4);                  // sort key + rank
-    b += Lo.np * 9 * 8;                    // velocity gradient of the last step
     b += Lo.nt * (nacc * 8 + nout * esz);  // grid
     b += Lo.nblk_total * (4 + 2 * 4);      // node-block flags + two lists
     b += Lo.nkeys * (4 + 4);               // histogram, offsets
@@ -239,6 +238,7 @@ struct mpm_gpu_handle {
     PBlk *plist = nullptr;
     int *npblk = nullptr;
     double *Lout = nullptr;
+    bool keep_L = true;  // MPM_OPT_KEEP_L: the last step of a call writes the velocity gradient
     unsigned long long *stats = nullptr;
     unsigned *rare = nullptr, *rare_n = nullptr;  // side list of the split particle passes
     unsigned *work = nullptr;                     // block counter of the particle passes
@@ -567,11 +567,12 @@ int graph_period(mpm_gpu_handle *h) {
     return MPM_OK;
 }
 
-// Launch-shape autotuner: times whole steps (HIP events) for candidate persistent-grid
-// sizes of the G2P (and fused) pass, then of the P2G pass, then of the grid kernel, and
-// keeps the fastest of each. The
-// steps it times are real steps of the run (results do not depend on launch shape beyond
-// atomic summation order). Returns the number of steps consumed.
+// Autotuner of the tile and bin sizes: times whole steps (HIP events) for candidate
+// persistent-grid sizes of the G2P (and fused) pass, then of the P2G pass, then of the grid
+// kernel, keeping the fastest of each, and then the sort interval (steps per binning of the
+// points into particle blocks). The steps it times are real steps of the run (results do not
+// depend on launch shape or sort interval beyond atomic summation order). Returns the
+// number of steps consumed.
 template <int DIM, class PT>
 int autotune(mpm_gpu_handle *h, int budget, int *used) {
     *used = 0;
@@ -622,6 +623,34 @@ int autotune(mpm_gpu_handle *h, int budget, int *used) {
         if (ms < best) { best = ms; best_grid = c; }
     }
     h->grid_wgs = best_grid;
+    // the sort interval (how many steps the particle-block bins serve before points are
+    // re-binned): one sort period of each candidate, its time per step, when the call has the
+    // steps to spare; the sort at the start of each period is forced
+    static const int sort_cand[] = {8, 16, 32};
+    if (!rc && h->sort_interval > 0 && budget - *used >= 8 + 16 + 32 + 1) {
+        float best_ps = 1e30f;
+        int best_si = h->sort_interval;
+        for (int c : sort_cand) {
+            h->sort_interval = c;
+            h->since_sort = c;
+            if (hipEventRecord(e0, h->stream) != hipSuccess) { rc = MPM_ERR_HIP; break; }
+            for (int r = 0; r < c && !rc; r++) {
+                rc = split_step<DIM, PT>(h, false, false);
+                (*used)++;
+            }
+            if (rc) break;
+            if (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = MPM_ERR_HIP;
+                break;
+            }
+            if (ms / c < best_ps) {
+                best_ps = ms / c;
+                best_si = c;
+            }
+        }
+        h->sort_interval = best_si;
+    }
     h->tuned = true;
     h->timing = timing;
     (void)hipEventDestroy(e0);
@@ -1113,6 +1142,14 @@ int mpm_gpu_set_option(mpm_gpu_handle *h, int key, int value) {
             }
             h->record = value != 0;
             return MPM_OK;
+        case MPM_OPT_KEEP_L:
+            h->keep_L = value != 0;
+            if (!h->keep_L && h->Lout) {
+                HIP_TRY(hipStreamSynchronize(h->stream));
+                HIP_TRY(hipFree(h->Lout));
+                h->Lout = nullptr;
+            }
+            return MPM_OK;
         case MPM_OPT_PASS_GRID:
         case MPM_OPT_P2G_GRID:
         case MPM_OPT_GRID_WGS:
@@ -1138,6 +1175,7 @@ int mpm_gpu_get_option(mpm_gpu_handle *h, int key, int *value) {
         case MPM_OPT_NODE_RECORD: *value = h->record; return MPM_OK;
         case MPM_OPT_GRID_WGS: *value = h->grid_wgs; return MPM_OK;
         case MPM_OPT_GRAPH_CAPTURES: *value = h->graph_captures; return MPM_OK;
+        case MPM_OPT_KEEP_L: *value = h->keep_L; return MPM_OK;
         default: return fail(MPM_ERR_ARG, "unknown option %d", key);
     }
 }
@@ -1177,7 +1215,7 @@ static int run_steps(mpm_gpu_handle *h, const double *dt, int nsim, int nsteps) 
                                h->stream));
         h->sims_dirty = false;
     }
-    if (!h->Lout) {
+    if (!h->Lout && h->keep_L) {
         HIP_TRY(hipMalloc(&h->Lout, sizeof(double) * 9 * std::max(1ll, h->lo.np)));
         HIP_TRY(hipMemsetAsync(h->Lout, 0, sizeof(double) * 9 * std::max(1ll, h->lo.np), h->stream));
     }
@@ -1289,7 +1327,8 @@ int mpm_gpu_download_aux(mpm_gpu_handle *h, int body, double *u, double *L) {
         for (long long i = 0; i < (long long)dim * n; i++) u[i] -= x0[i];
     }
     if (L) {
-        if (!h->Lout) {
+        if (!h->keep_L) return fail(MPM_ERR_STATE, "the velocity gradient is not kept (MPM_OPT_KEEP_L is 0)");
+        if (!h->Lout) {  // no step yet
             std::memset(L, 0, sizeof(double) * 9 * n);
             return MPM_OK;
         }

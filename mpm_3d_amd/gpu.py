@@ -39,6 +39,7 @@ MPM_OPT_GRID_WGS = 6
 MPM_OPT_GRAPH_CAPTURES = 7
 MPM_OPT_P2G_GRID = 8
 MPM_OPT_NODE_RECORD = 9
+MPM_OPT_KEEP_L = 10
 
 
 class MpmGpuError(RuntimeError):

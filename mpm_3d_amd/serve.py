@@ -2,24 +2,37 @@
 admission by device footprint, continuous batching.
 
 Every slot of the pool is a simulation of the batch handle with a capacity (grid N^3,
-points per body, one contact); a request occupies a free slot (``mpm_gpu_load_sim`` +
-uploads), all occupied slots advance together in chunks of steps (``mpm_gpu_step_batch``,
-each slot with its own dt), and a slot whose request has run its steps is downloaded and
-freed while the others keep going. Requests that arrive while others are on the GPU wait
-only for a free slot. The number of slots is what fits a device-memory budget
-(``mpm_gpu_footprint`` of one slot).
+points per body, one contact); slots come in classes of different capacity, so that the
+device memory a slot holds follows the requests it is meant for. A request occupies the
+smallest free slot that fits it (``mpm_gpu_load_sim`` + uploads), all occupied slots
+advance together in chunks of steps (``mpm_gpu_step_batch``, each slot with its own dt),
+and a slot whose request has run its steps is downloaded and freed while the others keep
+going. Requests that arrive while others are on the GPU wait only for a free slot that
+fits them. The reference has no serving layer: each request stands for a Job of its own
+(config.cpp:176-213, job.hpp:76-105) stepped by ExplicitUSL::step (explicit_usl.cpp:48-87).
+
+Failure handling, per request (the others are unaffected):
+  refused    no slot of the pool can ever hold it (grid, bodies or points above every
+             slot's capacity); it is never loaded
+  cancelled  ``cancel_at`` reached while it waited (dropped from the queue) or ran (its
+             slot is freed at the next chunk boundary)
+  failed     its state at completion holds a non-finite value
+  done       otherwise
+
+``run_stream`` drives any executor with the methods of ``SlotPool`` (n_slots, fits,
+fits_any, capacity, free_slots, load, active_slots, remaining, request_of, step, result,
+free), so the admission logic is tested on the CPU against a mock executor.
 """
 from __future__ import annotations
 
-import collections
 import dataclasses
 import time
-from typing import List, Optional, Sequence, Tuple
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
 from mpm_3d_amd import workloads as W
-from mpm_3d_amd.gpu import MPM_OPT_GRAPH, MPM_PREC_FP32, GpuSim, load_library, _sim_desc
+from mpm_3d_amd.gpu import MPM_OPT_GRAPH, MPM_OPT_KEEP_L, MPM_PREC_FP32, GpuSim, load_library, _sim_desc
 
 
 @dataclasses.dataclass
@@ -31,6 +44,14 @@ class CapBody:
     boundary: bool = True
 
 
+@dataclasses.dataclass
+class SlotClass:
+    """`count` slots of grid N_cap^dim holding up to body_caps[b] points in body b."""
+    count: int
+    N_cap: int
+    body_caps: Tuple[int, ...]
+
+
 def slot_workload(dim, N_cap, body_caps, contact=True):
     bodies = [CapBody(int(c)) for c in body_caps]
     contacts = [W.Contact(0, 1, 0.4)] if contact and len(bodies) > 1 else []
@@ -38,36 +59,106 @@ def slot_workload(dim, N_cap, body_caps, contact=True):
     return W.Workload("slot", dim, (1.0,) * dim, (N_cap,) * dim, bodies, contacts, 1e-5, 0, g)
 
 
-def slots_for_budget(dim, N_cap, body_caps, budget_bytes, precision=MPM_PREC_FP32):
-    """How many slots of this capacity fit the byte budget (device footprint)."""
+def slot_footprint(dim, N_cap, body_caps, precision=MPM_PREC_FP32):
+    """Device bytes of one slot of this capacity (mpm_gpu_footprint)."""
     lib = load_library()
     keep = []
     d = _sim_desc(slot_workload(dim, N_cap, body_caps), keep)
     per = lib.mpm_gpu_footprint(d, 1, precision)
     if per <= 0:
         raise ValueError("footprint query failed")
+    return int(per)
+
+
+def slots_for_budget(dim, N_cap, body_caps, budget_bytes, precision=MPM_PREC_FP32):
+    """How many slots of this capacity fit the byte budget (device footprint)."""
+    per = slot_footprint(dim, N_cap, body_caps, precision)
     return max(1, int(budget_bytes // per)), per
 
 
+def request_fits(w, N_cap, body_caps, dim=3):
+    """Whether workload w fits a slot of grid N_cap and these body capacities."""
+    if w.dim != dim or any(n > N_cap for n in w.N) or len(w.bodies) > len(body_caps):
+        return False
+    if len(w.contacts) > (1 if len(body_caps) > 1 else 0):
+        return False
+    return all(b.n <= c for b, c in zip(w.bodies, body_caps))
+
+
+def classes_for_budget(dim, class_shapes, budget_bytes, precision=MPM_PREC_FP32, shares=None):
+    """Slot classes sized by device footprint: class_shapes is a list of (N_cap, body_caps);
+    the byte budget is split between them by `shares` (default: equal), each class holding
+    as many slots as its share's bytes pay for (at least one)."""
+    shares = shares or [1.0] * len(class_shapes)
+    tot = float(sum(shares))
+    out, per = [], []
+    for (N_cap, caps), sh in zip(class_shapes, shares):
+        f = slot_footprint(dim, N_cap, caps, precision)
+        out.append(SlotClass(max(1, int(budget_bytes * sh / tot // f)), N_cap, tuple(caps)))
+        per.append(f)
+    return out, per
+
+
 class SlotPool:
-    def __init__(self, n_slots, dim=3, N_cap=128, body_caps=(2_000_000, 1_000_000), precision=MPM_PREC_FP32,
-                 graphs=True):
-        self.n_slots = n_slots
-        caps = [slot_workload(dim, N_cap, body_caps) for _ in range(n_slots)]
+    """Slots in one batch handle. Either `n_slots` slots of one capacity (N_cap, body_caps)
+    or `classes`, a list of SlotClass. keep_L=False: no velocity-gradient plane (72 B per
+    point slot), which only mpm_gpu_download_aux reads."""
+
+    def __init__(self, n_slots=None, dim=3, N_cap=128, body_caps=(2_000_000, 1_000_000), precision=MPM_PREC_FP32,
+                 graphs=True, classes=None, keep_L=False):
+        if classes is None:
+            classes = [SlotClass(int(n_slots), N_cap, tuple(body_caps))]
+        self.dim = dim
+        self.classes = list(classes)
+        self.cap = []  # (N_cap, body_caps) per slot
+        caps = []
+        for c in self.classes:
+            for _ in range(c.count):
+                caps.append(slot_workload(dim, c.N_cap, c.body_caps))
+                self.cap.append((c.N_cap, tuple(c.body_caps)))
+        self.n_slots = len(caps)
+        self.footprint = [slot_footprint(dim, n, b, precision) for n, b in self.cap]
         self.sim = GpuSim(caps, precision=precision, upload=False)
+        if not keep_L:
+            self.sim.set_option(MPM_OPT_KEEP_L, 0)
         if graphs:
             self.sim.set_option(MPM_OPT_GRAPH, 1)
-        self.req = [None] * n_slots   # request index per slot
-        self.loaded = [None] * n_slots  # workload per slot
-        self.left = [0] * n_slots     # steps still to run
-        self.dt = [1e-5] * n_slots
+        self.req = [None] * self.n_slots     # request index per slot
+        self.loaded = [None] * self.n_slots  # workload per slot
+        self.left = [0] * self.n_slots       # steps still to run
+        self.dt = [1e-5] * self.n_slots
+
+    # -- executor interface used by run_stream ------------------------------------
+    def fits(self, s, w):
+        return request_fits(w, self.cap[s][0], self.cap[s][1], self.dim)
+
+    def fits_any(self, w):
+        return any(self.fits(s, w) for s in range(self.n_slots))
+
+    def capacity(self, s):
+        """Ordering key of slot s: its device footprint (smaller slots are taken first)."""
+        return self.footprint[s]
 
     def free_slots(self):
         return [s for s in range(self.n_slots) if self.req[s] is None]
 
+    def active_slots(self):
+        return [s for s in range(self.n_slots) if self.req[s] is not None]
+
+    def remaining(self, s):
+        return self.left[s]
+
+    def request_of(self, s):
+        return self.req[s]
+
     def load(self, s, r, w, steps):
         self.sim.load_sim(s, w)
         self.req[s], self.left[s], self.dt[s], self.loaded[s] = r, steps, w.dt, w
+
+    def step(self, k):
+        self.sim.step_batch(self.dt, k)
+        for s in self.active_slots():
+            self.left[s] -= k
 
     def result(self, s):
         b0 = self.sim.body0[s]
@@ -85,60 +176,120 @@ class SlotPool:
         self.sim.close()
 
 
-def run_stream(pool: SlotPool, requests: Sequence[Tuple[W.Workload, int]], arrivals: Sequence[float],
-               chunk_max=64, keep_results=False, log=None):
+def check_result(w, bodies):
+    """None if a finished request's state is finite, else why it failed. (A point whose
+    position became NaN leaves the grid and is deactivated; its velocity keeps the NaN.)"""
+    for bi, st in enumerate(bodies):
+        for key in ("x", "xt", "T6", "v"):
+            a = st.get(key)
+            if a is not None and not np.all(np.isfinite(a)):
+                return f"non-finite {key} in body {bi}"
+    return None
+
+
+def run_stream(pool, requests: Sequence[Tuple[W.Workload, int]], arrivals: Sequence[float], chunk_max=64,
+               keep_results=False, log=None, cancel_at: Optional[Dict[int, float]] = None, clock=None,
+               sleep=None):
     """Serve `requests` arriving at `arrivals` (seconds from start, sorted). Returns per-
-    request (arrival, start, finish) times, the results if kept, and the makespan."""
+    request (arrival, start, finish) times and status, the results if kept, and the
+    makespan. Admission is first come, first served over the slots a request fits,
+    smallest slot first: a free slot goes to the earliest waiting request that fits it, so
+    a large request waiting for a large slot gets it before any later request."""
+    clock = clock or time.perf_counter
+    sleep = sleep or time.sleep
+    cancel_at = dict(cancel_at or {})
     n = len(requests)
-    t0 = time.perf_counter()
-    queue = collections.deque()
+    t0 = clock()
+    queue: List[int] = []
     nxt = 0
     start = [None] * n
     finish = [None] * n
+    status = ["pending"] * n
     results = [None] * n
     done = 0
     t_step = t_admin = 0.0
     chunks = 0
+
+    def close(r, st):
+        nonlocal done
+        status[r] = st
+        finish[r] = clock() - t0
+        done += 1
+
     while done < n:
-        now = time.perf_counter() - t0
+        now = clock() - t0
         while nxt < n and arrivals[nxt] <= now:
-            queue.append(nxt)
+            w = requests[nxt][0]
+            if pool.fits_any(w):
+                queue.append(nxt)
+            else:
+                close(nxt, "refused")
+                if log:
+                    log(f"request {nxt} refused: larger than every slot")
             nxt += 1
-        ta = time.perf_counter()
-        for s in pool.free_slots():
-            if not queue:
+        ta = clock()
+        # cancellations due: waiting requests leave the queue, running ones free their slot
+        for r, tc in list(cancel_at.items()):
+            if tc > now or status[r] != "pending":
+                continue
+            if r in queue:
+                queue.remove(r)
+                close(r, "cancelled")
+                del cancel_at[r]
+            else:
+                for s in pool.active_slots():
+                    if pool.request_of(s) == r:
+                        pool.free(s)
+                        close(r, "cancelled")
+                        del cancel_at[r]
+                        break
+        # admission: in queue order, each request into the smallest free slot it fits;
+        # one that fits none waits, and later ones may take what is left
+        free = sorted(pool.free_slots(), key=pool.capacity)
+        for r in list(queue):
+            if not free:
                 break
-            r = queue.popleft()
             w, steps = requests[r]
-            pool.load(s, r, w, steps)
-            start[r] = time.perf_counter() - t0
-        t_admin += time.perf_counter() - ta
-        active = [s for s in range(pool.n_slots) if pool.req[s] is not None]
+            cand = [s for s in free if pool.fits(s, w)]
+            if cand:
+                pool.load(cand[0], r, w, steps)
+                start[r] = clock() - t0
+                queue.remove(r)
+                free.remove(cand[0])
+        t_admin += clock() - ta
+        active = pool.active_slots()
         if not active:
             if nxt < n:
-                time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+                wait = arrivals[nxt] - (clock() - t0)
+                nexts = [tc for r, tc in cancel_at.items() if status[r] == "pending"]
+                if nexts:
+                    wait = min(wait, min(nexts) - (clock() - t0))
+                if wait > 0:
+                    sleep(wait)
+            elif queue:
+                raise RuntimeError("requests wait for slots while none are in use")
             continue
-        k = min(chunk_max, min(pool.left[s] for s in active))
-        ts = time.perf_counter()
-        pool.sim.step_batch(pool.dt, k)
-        t_step += time.perf_counter() - ts
+        k = min(chunk_max, min(pool.remaining(s) for s in active))
+        ts = clock()
+        pool.step(k)
+        t_step += clock() - ts
         chunks += 1
-        ta = time.perf_counter()
+        ta = clock()
         for s in active:
-            pool.left[s] -= k
-            if pool.left[s] == 0:
-                r = pool.req[s]
+            if pool.remaining(s) == 0:
+                r = pool.request_of(s)
+                res = pool.result(s)
+                why = check_result(requests[r][0], res)
                 if keep_results:
-                    results[r] = pool.result(s)
-                finish[r] = time.perf_counter() - t0
+                    results[r] = res
                 pool.free(s)
-                done += 1
+                close(r, "done" if why is None else f"failed: {why}")
                 if log:
-                    log(f"request {r} done: {requests[r][0].n_points} points x {requests[r][1]} steps, "
+                    log(f"request {r} {status[r]}: {requests[r][0].n_points} points x {requests[r][1]} steps, "
                         f"latency {finish[r] - arrivals[r]:.3f} s")
-        t_admin += time.perf_counter() - ta
-    return {"arrival": list(arrivals), "start": start, "finish": finish, "results": results,
-            "makespan": time.perf_counter() - t0, "step_s": t_step, "admin_s": t_admin, "chunks": chunks}
+        t_admin += clock() - ta
+    return {"arrival": list(arrivals), "start": start, "finish": finish, "status": status, "results": results,
+            "makespan": clock() - t0, "step_s": t_step, "admin_s": t_admin, "chunks": chunks}
 
 
 def poisson_arrivals(n, rate, seed=0xC0FFEE + 5):

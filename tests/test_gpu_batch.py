@@ -12,7 +12,8 @@ import pytest
 from mpm_3d_amd import serve as S
 from mpm_3d_amd import workloads as W
 from mpm_3d_amd.gpu import (MPM_OPT_AUTOTUNE, MPM_OPT_GRAPH, MPM_OPT_GRAPH_CAPTURES, MPM_OPT_GRID_WGS,
-                            MPM_OPT_P2G_GRID, MPM_OPT_PASS_GRID, MPM_PREC_FP32, MPM_PREC_FP64, GpuSim)
+                            MPM_OPT_P2G_GRID, MPM_OPT_PASS_GRID, MPM_OPT_SORT_INTERVAL, MPM_PREC_FP32, MPM_PREC_FP64,
+                            GpuSim)
 from tests._harness import oracle_of
 
 pytestmark = pytest.mark.gpu
@@ -82,6 +83,26 @@ def test_autotune_keeps_results():
     assert b.get_option(MPM_OPT_PASS_GRID) in (512, 1024, 2048, 4096, 8192)
     assert b.get_option(MPM_OPT_P2G_GRID) in (768, 1024, 1536, 2048, 4096)
     assert b.get_option(MPM_OPT_GRID_WGS) in (512, 2048, 8192)
+    for i, w in enumerate(ws):
+        ex, ev = _err(w, a.points(i), b.points(i))
+        assert ex <= 1e-9 and ev <= 1e-8
+
+
+def test_autotune_sort_interval_keeps_results():
+    """A call long enough to tune the sort interval too (8, 16 or 32 steps per binning), with
+    graph replay after it: the same state as a handle that never tuned, to rounding."""
+    ws = _small_ensemble()
+    a = GpuSim(ws, precision=MPM_PREC_FP64)
+    b = GpuSim(ws, precision=MPM_PREC_FP64)
+    b.set_option(MPM_OPT_AUTOTUNE, 1)
+    b.set_option(MPM_OPT_GRAPH, 1)
+    dts = [w.dt for w in ws]
+    a.step_batch(dts, 150)
+    b.step_batch(dts, 150)
+    assert b.get_option(MPM_OPT_AUTOTUNE) == 2
+    assert b.get_option(MPM_OPT_SORT_INTERVAL) in (8, 16, 32)
+    b.step_batch(dts, 70)
+    a.step_batch(dts, 70)
     for i, w in enumerate(ws):
         ex, ev = _err(w, a.points(i), b.points(i))
         assert ex <= 1e-9 and ev <= 1e-8

@@ -136,3 +136,33 @@ def test_state_errors():
     assert abs(m.sum() - w.bodies[0].m.sum()) <= 1e-12 * w.bodies[0].m.sum()
     g.close()
 
+
+
+def test_keep_L_off_refuses_the_velocity_gradient():
+    """MPM_OPT_KEEP_L 0: no velocity-gradient plane; steps give the same state, the
+    displacement still downloads, and asking for L returns MPM_ERR_STATE."""
+    w = _workload()
+    a = gpu_of(w)
+    b = gpu_of(w)
+    b.set_option(G.MPM_OPT_KEEP_L, 0)
+    assert b.get_option(G.MPM_OPT_KEEP_L) == 0
+    a.step(w.dt, 12)
+    b.step(w.dt, 12)
+    h = 1.0 / 16
+    for gb in range(2):
+        assert np.max(np.abs(a.points(gb)["x"] - b.points(gb)["x"])) <= 1e-12 * h
+    n = w.bodies[0].n
+    u = np.zeros(3 * n)
+    L = np.zeros(9 * n)
+    lib = b.lib
+    assert lib.mpm_gpu_download_aux(b.h, 0, u.ctypes.data_as(DP), None) == 0
+    ua = a.aux(0)["u"]
+    assert np.max(np.abs(u.reshape(3, n).T - ua)) <= 1e-12 * h
+    _expect(lib, lib.mpm_gpu_download_aux(b.h, 0, None, L.ctypes.data_as(DP)), STATE, "MPM_OPT_KEEP_L is 0")
+    # turned back on, the next call writes L again
+    b.set_option(G.MPM_OPT_KEEP_L, 1)
+    b.step(w.dt, 1)
+    a.step(w.dt, 1)
+    assert np.max(np.abs(b.aux(0)["L"] - a.aux(0)["L"])) <= 1e-6 * max(1.0, np.max(np.abs(a.aux(0)["L"])))
+    a.close()
+    b.close()

@@ -112,6 +112,20 @@ def test_served_precision_matches_oracle(oracle_results):
             assert ex <= 1e-5 and ev <= 1e-5 and eT <= 1e-4 and eV <= 1e-5
 
 
+def test_served_slot_classes_match_oracle(oracle_results):
+    """Two slot classes: the 27,000-point drops fit only the large slot, the others take the
+    small slot first; every result against the oracle (FP64 pool, parity bounds)."""
+    reqs, ora = oracle_results
+    classes = [S.SlotClass(1, 40, (27000, 13824)), S.SlotClass(1, 39, (17576, 13824))]
+    pool = S.SlotPool(classes=classes, dim=3, precision=MPM_PREC_FP64)
+    assert pool.capacity(1) < pool.capacity(0)
+    out = S.run_stream(pool, reqs, [0.0] * len(reqs), chunk_max=16, keep_results=True)
+    pool.close()
+    assert out["status"] == ["done"] * len(reqs)
+    for r, (w, steps) in enumerate(reqs):
+        compare_states(w, _Ora(ora[r]), _Served(out["results"][r]), steps * w.dt, label=f"classes {w.name}")
+
+
 def test_reupload_next_to_a_stepping_sim():
     """A two-simulation batch steps; simulation 0 is then reloaded with fewer points than its
     capacity (its points go into the free slots of the layout simulation 1 still sorts)
