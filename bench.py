@@ -122,14 +122,21 @@ def cpu_baseline(threads):
 
 def run_form(w, form, args, settle=None, traffic_key="column_3d_16M"):
     """Time `args.steps` steps of one form; returns its summary."""
-    from mpm_3d_amd.gpu import MPM_PREC_FP32, MPM_PREC_FP64, GpuSim
+    from mpm_3d_amd.gpu import (MPM_OPT_AUTOTUNE, MPM_OPT_GRID_WGS, MPM_OPT_P2G_GRID, MPM_OPT_PASS_GRID,
+                                MPM_OPT_SORT_INTERVAL, MPM_PREC_FP32, MPM_PREC_FP64, GpuSim)
     prec = MPM_PREC_FP32 if form == "fp32" else MPM_PREC_FP64
     sim = GpuSim(w, precision=prec, sort_interval=args.sort_interval, fused=args.fused)
     # the column's first steps are faster than later ones (points spread over more dual cells
-    # as it deforms); `settle` untimed steps start the timed window in the steadier regime
+    # as it deforms); `settle` untimed steps start the timed window in the steadier regime.
+    # The autotuner picks the launch shapes and the sort interval on the first of them.
     settle = args.settle if settle is None else settle
+    if not args.no_autotune and settle + args.warmup >= 90:  # the tuning needs 84 steps of its own
+        sim.set_option(MPM_OPT_AUTOTUNE, 1)
     if settle + args.warmup:
         sim.step(w.dt, settle + args.warmup)
+    tuned = {"sort_interval": sim.get_option(MPM_OPT_SORT_INTERVAL), "pass_grid": sim.get_option(MPM_OPT_PASS_GRID),
+             "p2g_grid": sim.get_option(MPM_OPT_P2G_GRID), "grid_wgs": sim.get_option(MPM_OPT_GRID_WGS),
+             "autotuned": sim.get_option(MPM_OPT_AUTOTUNE) == 2}
     sim.synchronize()
     sim.set_timing(True)
     t0 = time.perf_counter()
@@ -166,6 +173,7 @@ def run_form(w, form, args, settle=None, traffic_key="column_3d_16M"):
         },
         "step_hbm_fraction": round(step_bytes / wall / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": kern,
+        "tuned": tuned,
     }
 
 
@@ -330,6 +338,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--sort-interval", type=int, default=16)
     ap.add_argument("--fused", action="store_true", help="one fused G2P+P2G pass per step instead of two passes")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="column and collision legs with the default launch shapes and sort interval")
     ap.add_argument("--legs", default="column,collision,batched,served",
                     help="comma list of: column (16M column, both forms), collision (256^3 two-block "
                          "collision, FP64), batched (512-sim 2D ensemble), served")
@@ -379,7 +389,8 @@ def main():
         "config": {"workload": w.name, "points": int(w.n_points), "grid": list(w.N), "dt": w.dt,
                    "timed_window": f"steps {args.settle + args.warmup + 1}..{args.settle + args.warmup + args.steps}",
                    "form": "precision (FP32 state, FP64 accumulation)" if main_form == "fp32" else "plain FP64",
-                   "sort_interval": args.sort_interval, "particle_pass": "fused" if args.fused else "split"},
+                   "sort_interval": r["tuned"]["sort_interval"], "autotuned": r["tuned"]["autotuned"],
+                   "particle_pass": "fused" if args.fused else "split"},
         "roofline": r["roofline"],
         "step_hbm_fraction": r["step_hbm_fraction"],
         "hbm_copy_gbps": round(hbm_copy, 1),

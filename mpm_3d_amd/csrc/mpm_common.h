@@ -189,6 +189,9 @@ struct KArgs {
     unsigned *rare_n;
     // block counter of the particle passes ([0] next block, [1] finished workgroups)
     unsigned *work;
+    // per list parity: the grid update wrote a non-finite v or dv (set by k_grid, cleared by
+    // the next update)
+    unsigned *nonfin;
 };
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -306,6 +309,11 @@ template <int DIM, class R = double> struct Win {
     int base[DIM];
     R sw[DIM][3];
     R dw[DIM][3];
+    // per dimension, bit a set when slot a is a node of a corner's element (of the point's
+    // own element with CPDI off): the slots the reference's tuples cover. The other slot
+    // holds exact zeros and must not be multiplied into anything, so that a non-finite node
+    // or point value reaches exactly what the reference's tuples reach (0 * NaN is NaN).
+    unsigned char rch[DIM];
 };
 
 // cpdi_out tells the tuple path whether the point uses its corners (CPDI on) or not.
@@ -384,8 +392,18 @@ __host__ __device__ __forceinline__ int point_window(const SimDev &s, const doub
         }
         w.sw[d][0] = s0; w.sw[d][1] = s1; w.sw[d][2] = s2;
         w.dw[d][0] = d0; w.dw[d][1] = d1; w.dw[d][2] = d2;
+        w.rch[d] = (unsigned char)((vm ? (em == b ? 3 : 6) : 0) | (vp ? (ep == b ? 3 : 6) : 0));
     }
     return mode;
+}
+
+// whether window slot (a0, a1[, a2]) is one the reference's tuples cover (Win::rch)
+template <int DIM, class R>
+__host__ __device__ __forceinline__ bool slot_reached(const Win<DIM, R> &w, const int *off) {
+    unsigned m = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) m &= (unsigned)w.rch[d] >> off[d];
+    return m != 0;
 }
 
 // Per-tuple evaluation, exactly the reference's COO construction for one point

@@ -202,6 +202,7 @@ long long footprint_bytes(const Layout &Lo, int precision) {
     b += Lo.nkeys * (4 + 4);               // histogram, offsets
     b += Lo.npblk_total * sizeof(PBlk);    // block list
     b += Lo.np * 4 + 16;                   // side list of the particle passes
+    b += 8;                                // block counter and non-finite flags
     return b;
 }
 
@@ -242,6 +243,7 @@ struct mpm_gpu_handle {
     unsigned long long *stats = nullptr;
     unsigned *rare = nullptr, *rare_n = nullptr;  // side list of the split particle passes
     unsigned *work = nullptr;                     // block counter of the particle passes
+    unsigned *nonfin = nullptr;                   // non-finite grid output, per list parity
     int grid_nbm = MAXB;  // bodies per simulation the grid kernel is sized for (1, 2 or MAXB)
     bool record = false;      // MPM_OPT_NODE_RECORD
     double *rec = nullptr;    // node record planes (KArgs::rec)
@@ -308,6 +310,7 @@ KArgs kargs(mpm_gpu_handle *h) {
     A.rare = h->rare;
     A.rare_n = h->rare_n;
     A.work = h->work;
+    A.nonfin = h->nonfin;
     A.rec = nullptr;  // set by launch_grid when recording
     A.crec = nullptr;
     return A;
@@ -774,6 +777,7 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     TRY(dalloc(&h->rare, L.np));
     TRY(dalloc(&h->rare_n, 4));
     TRY(dalloc(&h->work, 2));
+    TRY(dalloc(&h->nonfin, 2));
     auto alloc_scan_tmp = [&](long long n, std::vector<unsigned *> &tmp) -> int {
         for (; n > 4096;) {
             const long long nb = ceil_div(n, 4096);
@@ -808,6 +812,7 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     if (e == hipSuccess) e = hipMemsetAsync(h->stats, 0, sizeof(unsigned long long) * 4, h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->rare_n, 0, sizeof(unsigned) * 4, h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->work, 0, sizeof(unsigned) * 2, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->nonfin, 0, sizeof(unsigned) * 2, h->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(h->d_sims, L.sims.data(), sizeof(SimDev) * L.sims.size(), hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess)
@@ -1719,7 +1724,7 @@ int mpm_gpu_destroy(mpm_gpu_handle *h) {
     void *ptrs[] = {h->d_sims, h->d_bodies, h->d_contacts, h->P, h->P2, h->Pw, h->Pw2, h->act, h->act2, h->pid,
                     h->pid2, h->pkey, h->prank, h->acc, h->out, h->out32, h->nflag, h->nlist, h->ncount, h->bcount, h->boff,
                     h->plist, h->npblk, h->Lout, h->stats, h->stage, h->rec, h->crec, h->rare, h->rare_n,
-                    h->work};
+                    h->work, h->nonfin};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (unsigned *p : h->scan_tmp)

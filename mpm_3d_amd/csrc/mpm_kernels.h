@@ -107,13 +107,13 @@ __host__ __device__ __forceinline__ void p2g_node_values(const P2GVals<DIM, R> &
     for (int d = 0; d < DIM; d++) out[1 + 2 * DIM + d] = b1 ? G[d] * q.m : (R)0;
 }
 
-// Visit the 3^DIM window slot by slot with the loop kept rolled (weights picked by
-// selects); used on the rare global-memory path so its loads do not set the kernel's
-// register budget. fn(off, S, G) with off the slot offsets from w.base.
+// Visit the window's slots the reference's tuples cover, one by one with the loop kept
+// rolled (weights picked by selects); used on the rare global-memory path so its loads do
+// not set the kernel's register budget. fn(off, S, G) with off the slot offsets from w.base.
 template <class R>
 __host__ __device__ __forceinline__ R pick3(const R *a, int i) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
 
-template <int DIM, class R, class Fn>
+template <int DIM, class R, bool REACH = true, class Fn>
 __host__ __device__ __forceinline__ void for_each_sep(const Win<DIM, R> &w, R ext, Fn &&fn) {
     const R gscale = (R)1 / ((R)(1 << DIM) * (R)0.5 * ext);
     const R sscale = (R)1 / (R)(1 << DIM);
@@ -121,6 +121,8 @@ __host__ __device__ __forceinline__ void for_each_sep(const Win<DIM, R> &w, R ex
 #pragma unroll 1
     for (int k = 0; k < NS; k++) {
         const int c0 = k % 3, c1 = (k / 3) % 3, c2 = k / 9;
+        const int cs[3] = {c0, c1, c2};
+        if (REACH && !slot_reached<DIM, R>(w, cs)) continue;
         const R sx = pick3(w.sw[0], c0), dx = pick3(w.dw[0], c0);
         const R sy = pick3(w.sw[1], c1), dy = pick3(w.dw[1], c1);
         const R sz = DIM == 3 ? pick3(w.sw[DIM - 1], c2) : (R)1;
@@ -274,8 +276,6 @@ __device__ __forceinline__ void side_done(unsigned *rare_n, int k) {
 #ifndef MPM_P2G_ORDER
 #define MPM_P2G_ORDER 2
 #endif
-// P2G skips the LDS atomics of window slots whose weight and gradient are exact zeros (adding
-// +-0 to an accumulator that starts at +0 changes nothing, so the sums are the same).
 #ifndef MPM_DYN_BLOCKS
 #define MPM_DYN_BLOCKS 1
 #endif
@@ -288,9 +288,6 @@ template <int DIM, bool DO_G2P, bool DO_P2G, bool FP32>
 constexpr bool dyn_blocks() {
     return DIM == 3 && DO_G2P != DO_P2G && (DO_P2G || !FP32);
 }
-#ifndef MPM_P2G_SKIP0
-#define MPM_P2G_SKIP0 1
-#endif
 // Lane of the P2G lane order for a point of in-block dual cell `dual` (KB = 2 or 3 bits per
 // dimension, 64 keys).
 template <int DIM, int ORDER>
@@ -349,7 +346,6 @@ template <int DIM, bool DO_G2P, bool DO_P2G, bool NRM, class PT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PassTune<DO_G2P, DO_P2G, PT::FP32>::waves)))
 void k_particles(KArgs A, int want_L) {
     constexpr int ORDER = (!DO_G2P && DO_P2G) ? MPM_P2G_ORDER : 0;
-    constexpr bool SKIP0 = MPM_P2G_SKIP0 != 0;
     constexpr bool DYN = MPM_DYN_BLOCKS != 0 && dyn_blocks<DIM, DO_G2P, DO_P2G, PT::FP32>();
     constexpr bool SIDE = side_pass<DO_G2P, DO_P2G, PT::FP32>();
     constexpr bool PREFETCH = PassTune<DO_G2P, DO_P2G, PT::FP32>::prefetch;
@@ -371,6 +367,8 @@ void k_particles(KArgs A, int want_L) {
     const int tid = threadIdx.x;
     const int nent = *A.npblk;
     const R sscale = (R)1 / (R)(1 << DIM);
+    // the grid update this pass gathers wrote a non-finite value (k_grid, parity par ^ 1)
+    const bool grid_nf = DO_G2P && A.nonfin[A.par ^ 1] != 0u;
     // Block order. DYN: workgroups take blocks from a counter, the next index fetched one
     // block ahead (e_sh double-buffered: a slot is rewritten only after every thread read it
     // behind the end-of-block barrier), so no workgroup idles while others have blocks left.
@@ -496,7 +494,12 @@ void k_particles(KArgs A, int want_L) {
                     A.act[p] = 0;
                     continue;
                 }
-                if (SIDE && (mode == MODE_TUP || (mode == MODE_SEP && !window_in_patch<DIM>(w, org)))) {
+                // After a grid update with a non-finite node value every point takes the slot-
+                // by-slot path over the slots the reference's tuples cover: the row-factored
+                // gather multiplies every slot, and 0 * NaN would reach points the reference
+                // spares.
+                const bool lds_path = mode == MODE_SEP && window_in_patch<DIM>(w, org) && !grid_nf;
+                if (SIDE && (mode == MODE_TUP || (mode == MODE_SEP && !lds_path))) {
                     side_push(A.rare, &A.rare_n[1], p);  // k_g2p_side updates it
                     continue;
                 }
@@ -507,7 +510,7 @@ void k_particles(KArgs A, int want_L) {
 #pragma unroll
                     for (int c = 0; c < DIM; c++) Lp[d][c] = 0;
                 }
-                if (mode == MODE_SEP && window_in_patch<DIM>(w, org)) {
+                if (lds_path) {
                     // row-factored gather: per row, U = sum sw0 v, DV = sum sw0 dv, LX = sum dw0 v
                     const R gscale = (R)1 / ((R)(1 << DIM) * (R)0.5 * q.ext);
                     const int i0 = patch_index<DIM>(w.base, org);
@@ -638,9 +641,11 @@ void k_particles(KArgs A, int want_L) {
 #pragma unroll
                         for (int c1 = 0; c1 < 3; c1++) {
                             // window slots no corner reaches hold exact zeros (weight and
-                            // gradient): their rows and nodes add nothing and are skipped
-                            if (SKIP0 && !(w.sw[1][c1] != (R)0 || w.dw[1][c1] != (R)0)) continue;
-                            if (SKIP0 && DIM == 3 && !(w.sw[DIM - 1][c2] != (R)0 || w.dw[DIM - 1][c2] != (R)0)) continue;
+                            // gradient): their rows and nodes add nothing and are skipped. (A
+                            // point reaching P2G has a finite position; a NaN node value in G2P
+                            // makes the position NaN too, and such points leave at the map.)
+                            if (!(w.sw[1][c1] != (R)0 || w.dw[1][c1] != (R)0)) continue;
+                            if (DIM == 3 && !(w.sw[DIM - 1][c2] != (R)0 || w.dw[DIM - 1][c2] != (R)0)) continue;
                             const RowW<D> rw = row_weights<DIM, R, D>(w, c1, c2, (D)sscale, gscale);
                             const D am = dm * rw.Su;
                             D ap[DIM], af0[DIM], af1[DIM];
@@ -657,7 +662,7 @@ void k_particles(KArgs A, int want_L) {
                             for (int c0 = 0; c0 < 3; c0++) {
                                 const int i = i0 + c0 + P * c1 + ZS * c2;
                                 const D sx = sx0[c0], dx = dx0[c0];
-                                if (SKIP0 && sx == 0.0 && dx == 0.0) continue;
+                                if (sx == 0.0 && dx == 0.0) continue;
                                 atomicAdd(&lacc[GF<DIM>::M * PSTR + i], am * sx);
 #pragma unroll
                                 for (int d = 0; d < DIM; d++) {
@@ -681,7 +686,7 @@ void k_particles(KArgs A, int want_L) {
                         hi[d] = w.base[d] + 2;
                     }
                     mark_range<DIM>(A, s, lo, hi);
-                    for_each_sep<DIM, R>(w, q.ext, [&](const int *off, R S, const R *G) {
+                    for_each_sep<DIM, R, false>(w, q.ext, [&](const int *off, R S, const R *G) {
                         int nijk[DIM];
 #pragma unroll
                         for (int d = 0; d < DIM; d++) nijk[d] = w.base[d] + off[d];
@@ -895,7 +900,10 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
     using G = GF<DIM>;
     constexpr int NB = Cfg<DIM>::NB, L = Cfg<DIM>::LNB;
     const int par = A.par;
-    if (blockIdx.x == 0 && threadIdx.x == 0) A.ncount[par ^ 1] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        A.ncount[par ^ 1] = 0;
+        A.nonfin[par ^ 1] = 0u;  // the flag of the update before this one
+    }
     const int cnt = A.ncount[par];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
@@ -1116,8 +1124,10 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
 #pragma unroll
                 for (int d = 0; d < DIM; d++) {
                     const double pn = p[b][d] + f[b][d] * dt;
-                    outp[(G::OV + d) * A.nt + gi] = (GO)(has ? pn * rm : 0.0);
-                    outp[(G::ODV + d) * A.nt + gi] = (GO)(has ? (f[b][d] * dt) * rm : 0.0);
+                    const GO vo = (GO)(has ? pn * rm : 0.0), dvo = (GO)(has ? (f[b][d] * dt) * rm : 0.0);
+                    outp[(G::OV + d) * A.nt + gi] = vo;
+                    outp[(G::ODV + d) * A.nt + gi] = dvo;
+                    if (!isfinite(vo) || !isfinite(dvo)) A.nonfin[par] = 1u;
                     if (A.rec) {
                         A.rec[(1 + d) * A.nt + gi] = pn;
                         A.rec[(1 + DIM + d) * A.nt + gi] = f[b][d];

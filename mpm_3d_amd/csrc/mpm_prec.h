@@ -318,6 +318,7 @@ struct Prec32 {
             }
             w.sw[d][0] = s0; w.sw[d][1] = s1; w.sw[d][2] = s2;
             w.dw[d][0] = d0; w.dw[d][1] = d1; w.dw[d][2] = d2;
+            w.rch[d] = (unsigned char)((vm ? (em == b ? 3 : 6) : 0) | (vp ? (ep == b ? 3 : 6) : 0));
         }
         return mode;
     }

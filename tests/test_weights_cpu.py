@@ -47,6 +47,9 @@ def _compare(dim, L, N, x, vol, cpdi):
     # slots outside the grid (padding nodes -1 and N+1) must carry exact zeros
     assert np.all(Sg[~inside] == 0.0) and np.all(Gg[~inside] == 0.0)
     ids = (ijk[inside] * np.array(strides[:dim])).sum(axis=1)
+    # the slots the kernels multiply are exactly the nodes of the reference's tuples (a slot no
+    # corner's element covers is skipped, so 0 * NaN cannot reach a node the reference spares)
+    assert set(ids.tolist()) == set(np.asarray(node).tolist()), (x, sorted(set(ids.tolist()) ^ set(np.asarray(node).tolist())))
     so = _per_node(node.tolist(), S)
     sg = _per_node(ids.tolist(), Sg[inside])
     h = min(L[d] / N[d] for d in range(dim))
