@@ -875,6 +875,8 @@ int64_t mpm_gpu_footprint(const mpm_sim_desc *sims, int nsim, int precision) {
     return footprint_bytes(lo, precision);
 }
 
+static int ensure_stage(mpm_gpu_handle *h, long long elems);
+
 int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v, const double *T,
                    const double *V, const double *m, const double *extent, const int32_t *active) {
     if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
@@ -910,79 +912,38 @@ int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v
             HIP_TRY(hipMemcpyAsync(h->act2, a8.data(), n, hipMemcpyHostToDevice, h->stream));
             HIP_TRY(hipStreamSynchronize(h->stream));
         }
+        // the body's fields as planes of n doubles in h->stage (x[dim] v[dim] T[NTS] V m ext),
+        // then into slots [0, n) of the second buffer in the form's representation
         const int NTS = dim == 3 ? 6 : 4;
-        if (f32) {
-            // precision form: cell + 32-bit fixed-point offset (x = h (cell + q 2^-32)),
-            // lnJ = 0 with V0 = V, FP32 fields
-            const SimDev &s = h->lo.sims[B.sim];
-            std::vector<uint32_t> pl((size_t)nf * n);
-            float *pf = reinterpret_cast<float *>(pl.data());
-            const int FQ = 1, FV = 1 + dim, FT = 1 + 2 * dim, FLNJ = 1 + 2 * dim + NTS;
-            for (long long i = 0; i < n; i++) {
-                int c[3] = {0, 0, 0};
-                for (int d = 0; d < dim; d++) {
-                    const double t = x[(long long)d * n + i] / s.h[d];
-                    double cf = std::floor(t);
-                    double fr = t - cf;  // in [0, 1)
-                    if (!(cf >= -1.0)) {  // far outside (or NaN): deactivated by the first sort
-                        cf = -1.0;
-                        fr = 0.0;
-                    } else if (cf > (double)s.N[d]) {
-                        cf = (double)s.N[d];
-                        fr = 0.5;
-                    }
-                    double qf = std::nearbyint(std::ldexp(fr, 32));
-                    if (qf >= 4294967296.0) {  // rounds up to the next node
-                        qf = 0.0;
-                        cf += 1.0;
-                    }
-                    c[d] = (int)cf;
-                    pl[(FQ + d) * n + i] = (uint32_t)qf;
-                    pf[(FV + d) * n + i] = (float)v[(long long)d * n + i];
-                }
-                pl[i] = dim == 3 ? pack_cell<3>(c) : pack_cell<2>(c);
-                for (int k = 0; k < NTS; k++) pf[(FT + k) * n + i] = T ? (float)T[(long long)k * n + i] : 0.0f;
-                pf[FLNJ * n + i] = 0.0f;
-                pf[(FLNJ + 1) * n + i] = (float)V[i];
-                pf[(FLNJ + 2) * n + i] = (float)m[i];
-                pf[(FLNJ + 3) * n + i] = (float)extent[i];
-            }
-            for (int f = 0; f < nf; f++)
-                HIP_TRY(hipMemcpyAsync(h->Pw2 + (long long)f * np, pl.data() + (size_t)f * n,
-                                       sizeof(uint32_t) * n, hipMemcpyHostToDevice, h->stream));
-            HIP_TRY(hipStreamSynchronize(h->stream));
+        const int nsf = 2 * dim + NTS + 3;
+        int rc = ensure_stage(h, (long long)nsf * n);
+        if (rc) return rc;
+        double *st = h->stage;
+        auto put = [&](int plane, int count, const double *src) -> hipError_t {
+            return hipMemcpyAsync(st + (long long)plane * n, src, sizeof(double) * n * count, hipMemcpyHostToDevice,
+                                  h->stream);
+        };
+        HIP_TRY(put(0, dim, x));
+        HIP_TRY(put(dim, dim, v));
+        if (T) {
+            HIP_TRY(put(2 * dim, NTS, T));  // 6 planes in (xx, yy, zz, xy, xz, yz); 2D keeps the first four
         } else {
-            // this body's planes
-            using F3 = PF<3>;
-            using F2 = PF<2>;
-            const int FX = 0, FV = dim, FT = 2 * dim;
-            const int FVOL = dim == 3 ? F3::VOL : F2::VOL, FMASS = FVOL + 1, FEXT = FVOL + 2;
-            auto put = [&](int f, const double *src) -> hipError_t {
-                return hipMemcpyAsync(h->P2 + (long long)f * np, src, sizeof(double) * n, hipMemcpyHostToDevice,
-                                      h->stream);
-            };
-            for (int d = 0; d < dim; d++) {
-                HIP_TRY(put(FX + d, x + (long long)d * n));
-                HIP_TRY(put(FV + d, v + (long long)d * n));
-            }
-            std::vector<double> zeros;
-            for (int c = 0; c < NTS; c++) {
-                if (T) {
-                    // 6 planes in (xx, yy, zz, xy, xz, yz); 2D keeps the first four
-                    HIP_TRY(put(FT + c, T + (long long)c * n));
-                } else {
-                    if (zeros.empty()) zeros.assign(n, 0.0);
-                    HIP_TRY(put(FT + c, zeros.data()));
-                }
-            }
-            HIP_TRY(put(FVOL, V));
-            HIP_TRY(put(FMASS, m));
-            HIP_TRY(put(FEXT, extent));
-            if (zeros.empty()) zeros.assign(n, 0.0);
-            HIP_TRY(put(FEXT + 1, zeros.data()));  // gammap, gammap_dot start at 0
-            HIP_TRY(put(FEXT + 2, zeros.data()));
-            HIP_TRY(hipStreamSynchronize(h->stream));
+            const long long nz = (long long)NTS * n;
+            hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)ceil_div(nz, 256)), dim3(256), 0, h->stream,
+                               st + 2ll * dim * n, nz, 0.0);
         }
+        HIP_TRY(put(2 * dim + NTS, 1, V));
+        HIP_TRY(put(2 * dim + NTS + 1, 1, m));
+        HIP_TRY(put(2 * dim + NTS + 2, 1, extent));
+        const unsigned gi = (unsigned)ceil_div(n, 256);
+        if (dim == 3) {
+            if (f32) hipLaunchKernelGGL((k_import<3, Prec32, uint32_t>), dim3(gi), dim3(256), 0, h->stream, st, n, h->lo.sims[B.sim], h->Pw2, np);
+            else hipLaunchKernelGGL((k_import<3, Prec64, double>), dim3(gi), dim3(256), 0, h->stream, st, n, h->lo.sims[B.sim], h->P2, np);
+        } else {
+            if (f32) hipLaunchKernelGGL((k_import<2, Prec32, uint32_t>), dim3(gi), dim3(256), 0, h->stream, st, n, h->lo.sims[B.sim], h->Pw2, np);
+            else hipLaunchKernelGGL((k_import<2, Prec64, double>), dim3(gi), dim3(256), 0, h->stream, st, n, h->lo.sims[B.sim], h->P2, np);
+        }
+        HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(k_free_count, dim3(gp), dim3(256), 0, h->stream, np, h->pid, h->inact_count);
         scan_exclusive(h, h->inact_count, h->inact_off, gp, h->scan_tmp_i);
         if (f32)
@@ -1087,6 +1048,10 @@ int mpm_gpu_load_sim(mpm_gpu_handle *h, int sim, const mpm_sim_desc *d) {
     if (!h->lo.contacts.empty())
         HIP_TRY(hipMemcpyAsync(h->d_contacts, h->lo.contacts.data(), sizeof(ContactDev) * h->lo.contacts.size(),
                                hipMemcpyHostToDevice, h->stream));
+    // the device copy of the descriptors follows at once, so that every kernel before the next
+    // step (connectivity, downloads) sees the new grid
+    HIP_TRY(hipMemcpyAsync(h->d_sims, h->lo.sims.data(), sizeof(SimDev) * h->lo.sims.size(), hipMemcpyHostToDevice,
+                           h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->sims_dirty = true;
     return MPM_OK;
@@ -1254,66 +1219,32 @@ int mpm_gpu_download(mpm_gpu_handle *h, int body, double *x, double *v, double *
     const int dim = h->lo.dim;
     const long long n = h->count[body], np = h->lo.np;
     if (n == 0) return MPM_OK;
-    const bool f32 = h->precision == MPM_PREC_FP32;
-    const int nf = f32 ? (dim == 3 ? PW<3>::N : PW<2>::N) : (dim == 3 ? PF<3>::N : PF<2>::N);
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    int rc = ensure_stage(h, np * nf);
+    // one pass over the slots: the body's points write their state in original order into
+    // planes of n doubles (x, v, T, V) followed by the activity flags
+    const long long nd = (2ll * dim + 7) * n;
+    int rc = ensure_stage(h, nd + (n + 1) / 2);
     if (rc) return rc;
-    const int gp = (int)ceil_div(np, 256);
-    if (f32) {
-        uint32_t *st = reinterpret_cast<uint32_t *>(h->stage);
-        hipLaunchKernelGGL(k_unsort<uint32_t>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->Pw, np, st, h->pid);
-        HIP_TRY(hipGetLastError());
-        std::vector<uint32_t> pl((size_t)nf * n);
-        for (int f = 0; f < nf; f++)
-            HIP_TRY(hipMemcpyAsync(pl.data() + (size_t)f * n, st + (long long)f * np + B.p_off, sizeof(uint32_t) * n,
-                                   hipMemcpyDeviceToHost, h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        const float *pf = reinterpret_cast<const float *>(pl.data());
-        const SimDev &s = h->lo.sims[B.sim];
-        const int NTS = dim == 3 ? 6 : 4;
-        const int FQ = 1, FV = 1 + dim, FT = 1 + 2 * dim, FLNJ = 1 + 2 * dim + NTS;
-        for (long long i = 0; i < n; i++) {
-            int c[3] = {0, 0, 0};
-            if (dim == 3) unpack_cell<3>(pl[i], c);
-            else unpack_cell<2>(pl[i], c);
-            for (int d = 0; d < dim; d++) {
-                if (x) x[(long long)d * n + i] = s.h[d] * ((double)c[d] + std::ldexp((double)pl[(FQ + d) * n + i], -32));
-                if (v) v[(long long)d * n + i] = pf[(FV + d) * n + i];
-            }
-            if (T)
-                for (int k = 0; k < 6; k++) T[(long long)k * n + i] = k < NTS ? (double)pf[(FT + k) * n + i] : 0.0;
-            if (V) V[i] = (double)pf[(FLNJ + 1) * n + i] * std::exp((double)pf[FLNJ * n + i]);
-        }
+    KArgs A = kargs(h);
+    const unsigned gp = (unsigned)ceil_div(np, 256);
+    const bool f32 = h->precision == MPM_PREC_FP32;
+    if (dim == 3) {
+        if (f32) hipLaunchKernelGGL((k_export<3, Prec32>), dim3(gp), dim3(256), 0, h->stream, A, B.p_off, n, h->lo.sims[B.sim], h->stage);
+        else hipLaunchKernelGGL((k_export<3, Prec64>), dim3(gp), dim3(256), 0, h->stream, A, B.p_off, n, h->lo.sims[B.sim], h->stage);
     } else {
-        hipLaunchKernelGGL(k_unsort<double>, dim3(gp), dim3(256), 0, h->stream, np, nf, h->P, np, h->stage, h->pid);
-        HIP_TRY(hipGetLastError());
+        if (f32) hipLaunchKernelGGL((k_export<2, Prec32>), dim3(gp), dim3(256), 0, h->stream, A, B.p_off, n, h->lo.sims[B.sim], h->stage);
+        else hipLaunchKernelGGL((k_export<2, Prec64>), dim3(gp), dim3(256), 0, h->stream, A, B.p_off, n, h->lo.sims[B.sim], h->stage);
     }
-    auto get = [&](int f, double *dst) -> hipError_t {
-        return hipMemcpyAsync(dst, h->stage + (long long)f * np + B.p_off, sizeof(double) * n, hipMemcpyDeviceToHost,
-                              h->stream);
+    HIP_TRY(hipGetLastError());
+    auto get = [&](long long plane, int count, double *dst) -> hipError_t {
+        return hipMemcpyAsync(dst, h->stage + plane * n, sizeof(double) * n * count, hipMemcpyDeviceToHost, h->stream);
     };
-    const int FX = 0, FV = dim, FT = 2 * dim, NTS = dim == 3 ? 6 : 4;
-    const int FVOL = 2 * dim + NTS;
-    if (!f32) {
-        for (int d = 0; d < dim; d++) {
-            if (x) HIP_TRY(get(FX + d, x + (long long)d * n));
-            if (v) HIP_TRY(get(FV + d, v + (long long)d * n));
-        }
-        if (T) {
-            for (int c = 0; c < NTS; c++) HIP_TRY(get(FT + c, T + (long long)c * n));
-            for (int c = NTS; c < 6; c++) std::memset(T + (long long)c * n, 0, sizeof(double) * n);
-        }
-        if (V) HIP_TRY(get(FVOL, V));
-    }
-    if (active) {
-        int *tmp = nullptr;
-        HIP_TRY(hipMalloc(&tmp, sizeof(int) * np));
-        hipLaunchKernelGGL(k_unsort_act, dim3(gp), dim3(256), 0, h->stream, np, h->act, tmp, h->pid);
-        HIP_TRY(hipMemcpyAsync(active, tmp + B.p_off, sizeof(int) * n, hipMemcpyDeviceToHost, h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        HIP_TRY(hipFree(tmp));
-    }
+    if (x) HIP_TRY(get(0, dim, x));
+    if (v) HIP_TRY(get(dim, dim, v));
+    if (T) HIP_TRY(get(2 * dim, 6, T));
+    if (V) HIP_TRY(get(2 * dim + 6, 1, V));
+    if (active)
+        HIP_TRY(hipMemcpyAsync(active, reinterpret_cast<const int *>(h->stage + nd), sizeof(int) * n,
+                               hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     return MPM_OK;
 }
@@ -1337,14 +1268,13 @@ int mpm_gpu_download_aux(mpm_gpu_handle *h, int body, double *u, double *L) {
             std::memset(L, 0, sizeof(double) * 9 * n);
             return MPM_OK;
         }
-        int rc = ensure_stage(h, np * 9);
+        int rc = ensure_stage(h, n * 9);
         if (rc) return rc;
         const int gp = (int)ceil_div(np, 256);
-        hipLaunchKernelGGL(k_unsort<double>, dim3(gp), dim3(256), 0, h->stream, np, 9, h->Lout, np, h->stage, h->pid);
+        hipLaunchKernelGGL(k_unsort_range<double>, dim3(gp), dim3(256), 0, h->stream, np, 9, h->Lout, np, h->stage,
+                           h->pid, B.p_off, n);
         HIP_TRY(hipGetLastError());
-        for (int k = 0; k < 9; k++)
-            HIP_TRY(hipMemcpyAsync(L + (long long)k * n, h->stage + (long long)k * np + B.p_off, sizeof(double) * n,
-                                   hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipMemcpyAsync(L, h->stage, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, h->stream));
         HIP_TRY(hipStreamSynchronize(h->stream));
     }
     return MPM_OK;
@@ -1359,19 +1289,16 @@ int mpm_gpu_download_plastic(mpm_gpu_handle *h, int body, double *gammap, double
     if (n == 0) return MPM_OK;
     const bool f32 = h->precision == MPM_PREC_FP32;
     const int gpf = f32 ? (dim == 3 ? PW<3>::GP : PW<2>::GP) : (dim == 3 ? PF<3>::GP : PF<2>::GP);
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    int rc = ensure_stage(h, np * 2);
+    int rc = ensure_stage(h, n * 2);
     if (rc) return rc;
     const int gp = (int)ceil_div(np, 256);
     if (f32) {
         uint32_t *st = reinterpret_cast<uint32_t *>(h->stage);
-        hipLaunchKernelGGL(k_unsort<uint32_t>, dim3(gp), dim3(256), 0, h->stream, np, 2, h->Pw + (long long)gpf * np, np,
-                           st, h->pid);
+        hipLaunchKernelGGL(k_unsort_range<uint32_t>, dim3(gp), dim3(256), 0, h->stream, np, 2,
+                           h->Pw + (long long)gpf * np, np, st, h->pid, B.p_off, n);
         HIP_TRY(hipGetLastError());
         std::vector<float> g(2 * n);
-        for (int f = 0; f < 2; f++)
-            HIP_TRY(hipMemcpyAsync(g.data() + (size_t)f * n, st + (long long)f * np + B.p_off, sizeof(float) * n,
-                                   hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipMemcpyAsync(g.data(), st, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, h->stream));
         HIP_TRY(hipStreamSynchronize(h->stream));
         for (long long i = 0; i < n; i++) {
             if (gammap) gammap[i] = g[i];
@@ -1379,12 +1306,12 @@ int mpm_gpu_download_plastic(mpm_gpu_handle *h, int body, double *gammap, double
         }
         return MPM_OK;
     }
-    hipLaunchKernelGGL(k_unsort<double>, dim3(gp), dim3(256), 0, h->stream, np, 2, h->P + (long long)gpf * np, np,
-                       h->stage, h->pid);
+    hipLaunchKernelGGL(k_unsort_range<double>, dim3(gp), dim3(256), 0, h->stream, np, 2, h->P + (long long)gpf * np,
+                       np, h->stage, h->pid, B.p_off, n);
     HIP_TRY(hipGetLastError());
-    if (gammap) HIP_TRY(hipMemcpyAsync(gammap, h->stage + B.p_off, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    if (gammap) HIP_TRY(hipMemcpyAsync(gammap, h->stage, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
     if (gammap_dot)
-        HIP_TRY(hipMemcpyAsync(gammap_dot, h->stage + np + B.p_off, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipMemcpyAsync(gammap_dot, h->stage + n, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     return MPM_OK;
 }

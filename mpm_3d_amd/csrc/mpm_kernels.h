@@ -1207,6 +1207,40 @@ __global__ __launch_bounds__(256) void k_nodes_out(KArgs A, int si, long long gr
 }
 
 // ---------------------------------------------------------------------------------------
+// Downloads and uploads of one body. k_export: the slots whose original index lies in
+// [lo, lo + n) write their state as the C-ABI hands it out at index - lo of planes of n
+// doubles, x[DIM] v[DIM] T[6] V, and their activity flag into the ints after those planes.
+// k_import: staged points i < n (planes of n doubles: x[DIM] v[DIM] T[NTS] V m extent) into
+// slots [0, n) of the particle planes `dst` (np per plane) in the form's representation.
+// (The simulation's descriptor comes by value from the host: mpm_gpu_load_sim updates the
+// device copy only with the next step.)
+template <int DIM, class PT>
+__global__ __launch_bounds__(256) void k_export(KArgs A, long long lo, long long n, SimDev s, double *out) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= A.np) return;
+    const long long o = (long long)A.pid[p] - lo;
+    if (o < 0 || o >= n) return;
+    double x[DIM], v[DIM], T[6], V;
+    PT::template export_state<DIM>(A, s, p, x, v, T, V);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        out[d * n + o] = x[d];
+        out[(DIM + d) * n + o] = v[d];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) out[(2 * DIM + k) * n + o] = T[k];
+    out[(2 * DIM + 6) * n + o] = V;
+    reinterpret_cast<int *>(out + (2 * DIM + 7) * n)[o] = A.act[p];
+}
+
+template <int DIM, class PT, class E>
+__global__ __launch_bounds__(256) void k_import(const double *src, long long n, SimDev s, E *dst, long long np) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PT::template import_state<DIM>(src, n, i, s, dst, np);
+}
+
+// ---------------------------------------------------------------------------------------
 // connectivity of the current state (DefaultPoints::generateMap's element lookups), for the
 // points whose original index lies in [o_lo, o_lo + o_n); point o goes to entry o - o_lo
 template <int DIM, class PT>

@@ -142,6 +142,33 @@ struct Prec64 {
         for (int d = 0; d < DIM; d++) x[d] = A.P[(PF<DIM>::X + d) * A.np + p];
         ext = A.P[PF<DIM>::EXT * A.np + p];
     }
+
+    // slot p's state as the C-ABI hands it out: x, v (DIM each), T (6: xx yy zz xy xz yz,
+    // the components the form does not store as 0), V
+    template <int DIM>
+    __device__ __forceinline__ static void export_state(const KArgs &A, const SimDev &, long long p, double *x,
+                                                        double *v, double *T, double &V) {
+        using F = PF<DIM>;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            x[d] = A.P[(F::X + d) * A.np + p];
+            v[d] = A.P[(F::V + d) * A.np + p];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; c++) T[c] = c < F::NTS ? A.P[(F::T + c) * A.np + p] : 0.0;
+        V = A.P[F::VOL * A.np + p];
+    }
+
+    // point i < n of an upload, staged as planes of n doubles (x[DIM] v[DIM] T[NTS] V m ext),
+    // into slot i of the planes `dst` (np per plane); gammap and gammap_dot start at 0
+    template <int DIM>
+    __device__ __forceinline__ static void import_state(const double *src, long long n, long long i,
+                                                        const SimDev &, double *dst, long long np) {
+        using F = PF<DIM>;
+        for (int f = 0; f < F::EXT + 1; f++) dst[f * np + i] = src[f * n + i];
+        dst[F::GP * np + i] = 0.0;
+        dst[F::GPD * np + i] = 0.0;
+    }
 };
 
 struct Prec32 {
@@ -479,6 +506,68 @@ struct Prec32 {
         for (int d = 0; d < DIM; d++)
             x[d] = s.h[d] * ((double)c[d] + (double)A.Pw[(F::Q + d) * A.np + p] * 0x1p-32);
         ext = Pf[F::EXT * A.np + p];
+    }
+
+    // slot p's state as the C-ABI hands it out (see Prec64::export_state): x = h (c + q 2^-32)
+    // in FP64, V = V0 exp(lnJ) in FP64
+    template <int DIM>
+    __device__ __forceinline__ static void export_state(const KArgs &A, const SimDev &s, long long p, double *x,
+                                                        double *v, double *T, double &V) {
+        using F = PW<DIM>;
+        const float *Pf = reinterpret_cast<const float *>(A.Pw);
+        int c[DIM];
+        unpack_cell<DIM>(A.Pw[F::CELL * A.np + p], c);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            x[d] = s.h[d] * ((double)c[d] + (double)A.Pw[(F::Q + d) * A.np + p] * 0x1p-32);
+            v[d] = (double)Pf[(F::V + d) * A.np + p];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) T[k] = k < F::NTS ? (double)Pf[(F::T + k) * A.np + p] : 0.0;
+        V = (double)Pf[F::V0 * A.np + p] * exp((double)Pf[F::LNJ * A.np + p]);
+    }
+
+    // point i < n of an upload (staged as for Prec64::import_state) into the precision form:
+    // cell + 32-bit fixed-point offset (x = h (cell + q 2^-32)), lnJ = 0 with V0 = V, the rest
+    // rounded to FP32; a coordinate far outside the grid (or NaN) parks the point in cell -1,
+    // where the first sort deactivates it
+    template <int DIM>
+    __device__ __forceinline__ static void import_state(const double *src, long long n, long long i,
+                                                        const SimDev &s, uint32_t *dst, long long np) {
+        using F = PW<DIM>;
+        float *df = reinterpret_cast<float *>(dst);
+        int c[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const double t = src[d * n + i] / s.h[d];
+            double cf = floor(t);
+            double fr = t - cf;  // in [0, 1)
+            if (!(cf >= -1.0)) {
+                cf = -1.0;
+                fr = 0.0;
+            } else if (cf > (double)s.N[d]) {
+                cf = (double)s.N[d];
+                fr = 0.5;
+            }
+            double qf = rint(ldexp(fr, 32));
+            if (qf >= 4294967296.0) {  // rounds up to the next node
+                qf = 0.0;
+                cf += 1.0;
+            }
+            c[d] = (int)cf;
+            dst[(F::Q + d) * np + i] = (uint32_t)qf;
+            df[(F::V + d) * np + i] = (float)src[(DIM + d) * n + i];
+        }
+        dst[F::CELL * np + i] = pack_cell<DIM>(c);
+#pragma unroll
+        for (int k = 0; k < F::NTS; k++) df[(F::T + k) * np + i] = (float)src[(2 * DIM + k) * n + i];
+        const int o = 2 * DIM + F::NTS;
+        df[F::LNJ * np + i] = 0.0f;
+        df[F::V0 * np + i] = (float)src[o * n + i];
+        df[F::MASS * np + i] = (float)src[(o + 1) * n + i];
+        df[F::EXT * np + i] = (float)src[(o + 2) * n + i];
+        df[F::GP * np + i] = 0.0f;
+        df[F::GPD * np + i] = 0.0f;
     }
 };
 

@@ -180,16 +180,16 @@ __global__ __launch_bounds__(256) void k_bin_scatter(long long np, const E *P, E
     pid2[q] = o;
 }
 
-// slot order -> original order for selected planes (downloads); slots of unused capacity
-// (pid < 0) are skipped
+// the slots whose original index lies in [lo, lo + n): selected planes into planes of n
+// elements at index - lo (one body's points for a download)
 template <class E>
-__global__ __launch_bounds__(256) void k_unsort(long long np, int nf, const E *src, long long src_stride, E *dst,
-                                                const int *pid) {
+__global__ __launch_bounds__(256) void k_unsort_range(long long np, int nf, const E *src, long long src_stride,
+                                                      E *dst, const int *pid, long long lo, long long n) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
-    const int o = pid[p];
-    if (o < 0) return;
-    for (int f = 0; f < nf; f++) dst[f * np + o] = src[f * src_stride + p];
+    const long long o = (long long)pid[p] - lo;
+    if (o < 0 || o >= n) return;
+    for (int f = 0; f < nf; f++) dst[f * n + o] = src[f * src_stride + p];
 }
 
 __global__ __launch_bounds__(256) void k_fill_f64(double *a, long long n, double v) {
@@ -247,13 +247,6 @@ __global__ __launch_bounds__(256) void k_place(long long np, int nf, const E *st
     for (int f = 0; f < nf; f++) P[f * np + p] = stage[f * np + r];
     act[p] = act_stage[r];
     pid[p] = (int)(p_off + r);
-}
-
-__global__ __launch_bounds__(256) void k_unsort_act(long long np, const int8_t *act, int *dst, const int *pid) {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    const int o = pid[p];
-    if (o >= 0) dst[o] = act[p];
 }
 
 __global__ __launch_bounds__(256) void k_count_active(long long np, const int8_t *act, unsigned long long *out) {

@@ -208,6 +208,7 @@ def run_stream(pool, requests: Sequence[Tuple[W.Workload, int]], arrivals: Seque
     results = [None] * n
     done = 0
     t_step = t_admin = 0.0
+    t_load = t_result = t_free = 0.0  # parts of t_admin
     chunks = 0
 
     def close(r, st):
@@ -252,7 +253,9 @@ def run_stream(pool, requests: Sequence[Tuple[W.Workload, int]], arrivals: Seque
             w, steps = requests[r]
             cand = [s for s in free if pool.fits(s, w)]
             if cand:
+                tl = clock()
                 pool.load(cand[0], r, w, steps)
+                t_load += clock() - tl
                 start[r] = clock() - t0
                 queue.remove(r)
                 free.remove(cand[0])
@@ -278,18 +281,23 @@ def run_stream(pool, requests: Sequence[Tuple[W.Workload, int]], arrivals: Seque
         for s in active:
             if pool.remaining(s) == 0:
                 r = pool.request_of(s)
+                tr = clock()
                 res = pool.result(s)
                 why = check_result(requests[r][0], res)
+                t_result += clock() - tr
                 if keep_results:
                     results[r] = res
+                tf = clock()
                 pool.free(s)
+                t_free += clock() - tf
                 close(r, "done" if why is None else f"failed: {why}")
                 if log:
                     log(f"request {r} {status[r]}: {requests[r][0].n_points} points x {requests[r][1]} steps, "
                         f"latency {finish[r] - arrivals[r]:.3f} s")
         t_admin += clock() - ta
     return {"arrival": list(arrivals), "start": start, "finish": finish, "status": status, "results": results,
-            "makespan": clock() - t0, "step_s": t_step, "admin_s": t_admin, "chunks": chunks}
+            "makespan": clock() - t0, "step_s": t_step, "admin_s": t_admin, "chunks": chunks,
+            "load_s": t_load, "result_s": t_result, "free_s": t_free}
 
 
 def poisson_arrivals(n, rate, seed=0xC0FFEE + 5):
