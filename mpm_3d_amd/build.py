@@ -24,6 +24,7 @@ LIB = os.path.join(PKG, "lib")
 INC = os.path.join(ROOT, "include")
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")  # roctx ranges (rocprofiler-sdk-roctx)
 ARCH = "gfx950"
 
 # -ffp-contract=off: no implicit multiply-add fusion, so the connectivity arithmetic
@@ -49,7 +50,8 @@ def build_gpu(verbose=False):
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libmpm_gpu.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{CSRC}", "-o", tmp, os.path.join(CSRC, "mpm_gpu.hip")])
+    _run([HIPCC, *HIP_FLAGS, f"-I{INC}", f"-I{CSRC}", "-o", tmp, os.path.join(CSRC, "mpm_gpu.hip"),
+          f"-L{ROCM_LIB}", "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{ROCM_LIB}"])
     os.replace(tmp, out)
     return out
 

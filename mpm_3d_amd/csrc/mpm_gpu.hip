@@ -15,6 +15,8 @@
 #include <mutex>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "mpm_gpu.h"
 #include "mpm_kernels.h"
 #include "mpm_sort.h"
@@ -22,6 +24,15 @@
 using namespace mpm;
 
 namespace {
+
+// A roctx range over one C-ABI call (rocprofv3 --marker-trace shows uploads, steps,
+// downloads and slot reconfiguration beside the kernels they enqueue); no-ops without a tracer.
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
+};
 
 thread_local std::string g_err;
 
@@ -849,6 +860,7 @@ extern "C" {
 const char *mpm_gpu_last_error(void) { return g_err.c_str(); }
 
 int mpm_gpu_create_batch(const mpm_sim_desc *sims, int nsim, int precision, mpm_gpu_handle **out) {
+    TraceRange tr_("mpm_gpu_create_batch");
     if (!out) return fail(MPM_ERR_ARG, "out is NULL");
     *out = nullptr;
     Layout lo;
@@ -879,6 +891,7 @@ static int ensure_stage(mpm_gpu_handle *h, long long elems);
 
 int mpm_gpu_upload(mpm_gpu_handle *h, int body, const double *x, const double *v, const double *T,
                    const double *V, const double *m, const double *extent, const int32_t *active) {
+    TraceRange tr_("mpm_gpu_upload");
     if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
     if (body < 0 || body >= (int)h->lo.bodies.size()) return fail(MPM_ERR_ARG, "body %d out of range", body);
     const BodyDev &B = h->lo.bodies[body];
@@ -973,6 +986,7 @@ static int deactivate_body(mpm_gpu_handle *h, int b) {
 }
 
 int mpm_gpu_load_sim(mpm_gpu_handle *h, int sim, const mpm_sim_desc *d) {
+    TraceRange tr_("mpm_gpu_load_sim");
     if (!h || !d) return fail(MPM_ERR_ARG, "handle and descriptor are required");
     if (sim < 0 || sim >= (int)h->lo.sims.size()) return fail(MPM_ERR_ARG, "sim %d out of range", sim);
     SimDev &s = h->lo.sims[sim];
@@ -1058,6 +1072,7 @@ int mpm_gpu_load_sim(mpm_gpu_handle *h, int sim, const mpm_sim_desc *d) {
 }
 
 int mpm_gpu_clear_sim(mpm_gpu_handle *h, int sim) {
+    TraceRange tr_("mpm_gpu_clear_sim");
     if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
     if (sim < 0 || sim >= (int)h->lo.sims.size()) return fail(MPM_ERR_ARG, "sim %d out of range", sim);
     SimDev &s = h->lo.sims[sim];
@@ -1165,6 +1180,7 @@ int mpm_gpu_step_batch(mpm_gpu_handle *h, const double *dt, int nsim, int nsteps
 }
 
 static int run_steps(mpm_gpu_handle *h, const double *dt, int nsim, int nsteps) {
+    TraceRange tr_("mpm_gpu_step");
     if (nsteps < 0) return fail(MPM_ERR_ARG, "nsteps must be >= 0");
     if (nsim != (int)h->lo.sims.size()) return fail(MPM_ERR_ARG, "need %zu time steps, got %d", h->lo.sims.size(), nsim);
     for (int i = 0; i < nsim; i++)
@@ -1213,6 +1229,7 @@ static int ensure_stage(mpm_gpu_handle *h, long long elems) {
 }
 
 int mpm_gpu_download(mpm_gpu_handle *h, int body, double *x, double *v, double *T, double *V, int32_t *active) {
+    TraceRange tr_("mpm_gpu_download");
     if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
     if (body < 0 || body >= (int)h->lo.bodies.size()) return fail(MPM_ERR_ARG, "body %d out of range", body);
     const BodyDev &B = h->lo.bodies[body];
@@ -1392,6 +1409,7 @@ static void launch_nodes_out(mpm_gpu_handle *h, int si, long long grid_base, con
 extern "C" {
 
 int mpm_gpu_download_nodes(mpm_gpu_handle *h, int body, double *m, double *mxt, double *f, double *xt, double *dxt) {
+    TraceRange tr_("mpm_gpu_download_nodes");
     if (!h) return fail(MPM_ERR_ARG, "handle is NULL");
     if (body < 0 || body >= (int)h->lo.bodies.size()) return fail(MPM_ERR_ARG, "body %d out of range", body);
     if (!h->grid_valid) return fail(MPM_ERR_STATE, "no step since the last upload or load: no node values");
