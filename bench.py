@@ -278,7 +278,7 @@ def run_served(args, full_rate=None):
     (the 16M column's rate, the GPU full), which no schedule can beat."""
     from mpm_3d_amd import serve as S
     from mpm_3d_amd import workloads as W
-    from mpm_3d_amd.gpu import MPM_PREC_FP32
+    from mpm_3d_amd.gpu import MPM_OPT_SORT_INTERVAL, MPM_PREC_FP32
     reqs = W.serve_requests(args.serve_n)
     counts = [int(c) for c in args.serve_slots.split(",")]
     classes = [S.SlotClass(c, N, caps) for c, (N, caps) in zip(counts, SERVE_CLASSES)]
@@ -290,6 +290,7 @@ def run_served(args, full_rate=None):
     arrivals = S.poisson_arrivals(len(reqs), rate)
     res = S.run_stream(pool, reqs, arrivals, chunk_max=64)
     footprint = sum(pool.footprint)
+    sort_interval = pool.sim.get_option(MPM_OPT_SORT_INTERVAL)
     pool.close()
     lat = np.array([f - a for f, a in zip(res["finish"], res["arrival"])])
     print(f"[bench] served: {len(reqs)} requests on {pool.n_slots} slots ({footprint / 1e9:.1f} GB); capacity run "
@@ -308,7 +309,7 @@ def run_served(args, full_rate=None):
                     f"{min(w.n_points for w, _ in reqs)}..{max(w.n_points for w, _ in reqs)} points, "
                     f"500..5000 steps, {pu:.3e} particle-updates in all)",
         "slots": {"classes": [{"count": c.count, "N_cap": c.N_cap, "body_caps": list(c.body_caps)} for c in classes],
-                  "total": pool.n_slots, "footprint_bytes": footprint},
+                  "total": pool.n_slots, "footprint_bytes": footprint, "sort_interval": sort_interval},
         "statuses": {st: sat["status"].count(st) for st in set(sat["status"])},
         "capacity": {"solves_per_s": cap_solves, "particle_updates_per_s": pu / sat["makespan"],
                      "makespan_s": sat["makespan"], "stepping_s": sat["step_s"], "admission_s": sat["admin_s"],

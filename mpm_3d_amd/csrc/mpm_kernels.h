@@ -26,16 +26,19 @@
 
 namespace mpm {
 
-__device__ __forceinline__ void mark_block(const KArgs &A, long long gblk) {
-    if (atomicOr(&A.nflag[gblk], 1u) == 0u) {
+// A node block's flag holds one bit per body of its simulation that scattered into it this
+// step (bit b: the sim's body b); the first mark appends the block to the list of parity par.
+__device__ __forceinline__ void mark_block(const KArgs &A, long long gblk, unsigned bit) {
+    if (atomicOr(&A.nflag[gblk], bit) == 0u) {
         const int pos = atomicAdd(&A.ncount[A.par], 1);
         A.nlist[(long long)A.par * A.nblk_total + pos] = (int)gblk;
     }
 }
 
-// mark the node blocks covering nodes [lo, hi] (inclusive) of a sim; called by one thread
+// mark the node blocks covering nodes [lo, hi] (inclusive) of a sim for its body `lb`
+// (index within the sim); called by one thread
 template <int DIM>
-__device__ void mark_range(const KArgs &A, const SimDev &s, const int *lo, const int *hi) {
+__device__ void mark_range(const KArgs &A, const SimDev &s, int lbody, const int *lo, const int *hi) {
     constexpr int L = Cfg<DIM>::LNB;
     int b0[3] = {0, 0, 0}, b1[3] = {0, 0, 0};
 #pragma unroll
@@ -47,7 +50,7 @@ __device__ void mark_range(const KArgs &A, const SimDev &s, const int *lo, const
         for (int j = b0[1]; j <= b1[1]; j++)
             for (int i = b0[0]; i <= b1[0]; i++) {
                 long long lb = i + (long long)s.NBn[0] * (j + (long long)s.NBn[1] * k);
-                mark_block(A, s.nblk_off + lb);
+                mark_block(A, s.nblk_off + lb, 1u << lbody);
             }
 }
 
@@ -685,7 +688,7 @@ void k_particles(KArgs A, int want_L) {
                         lo[d] = w.base[d];
                         hi[d] = w.base[d] + 2;
                     }
-                    mark_range<DIM>(A, s, lo, hi);
+                    mark_range<DIM>(A, s, B.local, lo, hi);
                     for_each_sep<DIM, R, false>(w, q.ext, [&](const int *off, R S, const R *G) {
                         int nijk[DIM];
 #pragma unroll
@@ -708,7 +711,7 @@ void k_particles(KArgs A, int want_L) {
                                 atomicAdd(&lacc[f * PSTR + i], (double)vals[f]);
                             }
                         } else {
-                            mark_range<DIM>(A, s, nijk, nijk);
+                            mark_range<DIM>(A, s, B.local, nijk, nijk);
                             scatter_global(nijk, S, G);
                         }
                     });
@@ -754,7 +757,7 @@ void k_particles(KArgs A, int want_L) {
                 int hi[DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; d++) hi[d] = lo[d] + ex[d] - 1;
-                mark_range<DIM>(A, s, lo, hi);
+                mark_range<DIM>(A, s, B.local, lo, hi);
             }
         }
         __syncthreads();
@@ -820,7 +823,7 @@ __global__ __launch_bounds__(256) void k_p2g_side(KArgs A) {
                 lo[d] = w.base[d];
                 hi[d] = w.base[d] + 2;
             }
-            mark_range<DIM>(A, s, lo, hi);
+            mark_range<DIM>(A, s, B.local, lo, hi);
             for_each_sep<DIM, R>(w, q.ext, [&](const int *off, R S, const R *G) {
                 int nijk[DIM];
                 for (int d = 0; d < DIM; d++) nijk[d] = w.base[d] + off[d];
@@ -828,7 +831,7 @@ __global__ __launch_bounds__(256) void k_p2g_side(KArgs A) {
             });
         } else if (mode == MODE_TUP) {
             PT::template tuples<DIM>(s, q, cpdi, [&](const int *nijk, R S, const R *G) {
-                mark_range<DIM>(A, s, nijk, nijk);
+                mark_range<DIM>(A, s, B.local, nijk, nijk);
                 scatter(nijk, S, G);
             });
         }
@@ -946,6 +949,9 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
             wall = wall || ijk[d] == 0 || ijk[d] == s.N[d];
         }
         const long long addr = lb * 64 + lane;
+        // bodies of the sim that scattered into this block: the others hold zero accumulators
+        // here, which are not read (or re-zeroed); their outputs are still written (zeros)
+        const unsigned touched = NBM == 1 ? 1u : (unsigned)uniform((int)A.nflag[gblk]);
         if (valid) {
             const int nb = s.nbody;
             // f0: the force before contacts act, which the frictional box takes its normal
@@ -956,7 +962,7 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
                 m[b] = 0.0;
 #pragma unroll
                 for (int d = 0; d < DIM; d++) p[b][d] = f[b][d] = f0[b][d] = nr[b][d] = 0.0;
-                if (b < nb) {
+                if (b < nb && ((touched >> b) & 1u)) {
                     const BodyDev &B = A.bodies[s.body0 + b];
                     const long long gi = B.grid_base + addr;
                     m[b] = A.acc[G::M * A.nt + gi];

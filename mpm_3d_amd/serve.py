@@ -32,7 +32,8 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from mpm_3d_amd import workloads as W
-from mpm_3d_amd.gpu import MPM_OPT_GRAPH, MPM_OPT_KEEP_L, MPM_PREC_FP32, GpuSim, load_library, _sim_desc
+from mpm_3d_amd.gpu import (MPM_OPT_GRAPH, MPM_OPT_KEEP_L, MPM_OPT_SORT_INTERVAL, MPM_PREC_FP32, GpuSim, load_library,
+                             _sim_desc)
 
 
 @dataclasses.dataclass
@@ -102,10 +103,12 @@ def classes_for_budget(dim, class_shapes, budget_bytes, precision=MPM_PREC_FP32,
 class SlotPool:
     """Slots in one batch handle. Either `n_slots` slots of one capacity (N_cap, body_caps)
     or `classes`, a list of SlotClass. keep_L=False: no velocity-gradient plane (72 B per
-    point slot), which only mpm_gpu_download_aux reads."""
+    point slot), which only mpm_gpu_download_aux reads. sort_interval: steps between
+    re-sorts of the pool's points (32, the value the autotuner picks for the 16M column and
+    the 512-simulation ensemble; a pool's mix changes as it runs, so it is not tuned here)."""
 
     def __init__(self, n_slots=None, dim=3, N_cap=128, body_caps=(2_000_000, 1_000_000), precision=MPM_PREC_FP32,
-                 graphs=True, classes=None, keep_L=False):
+                 graphs=True, classes=None, keep_L=False, sort_interval=32):
         if classes is None:
             classes = [SlotClass(int(n_slots), N_cap, tuple(body_caps))]
         self.dim = dim
@@ -121,6 +124,7 @@ class SlotPool:
         self.sim = GpuSim(caps, precision=precision, upload=False)
         if not keep_L:
             self.sim.set_option(MPM_OPT_KEEP_L, 0)
+        self.sim.set_option(MPM_OPT_SORT_INTERVAL, int(sort_interval))
         if graphs:
             self.sim.set_option(MPM_OPT_GRAPH, 1)
         self.req = [None] * self.n_slots     # request index per slot
