@@ -314,6 +314,7 @@ def run_served(args, full_rate=None):
         "capacity": {"solves_per_s": cap_solves, "particle_updates_per_s": pu / sat["makespan"],
                      "makespan_s": sat["makespan"], "stepping_s": sat["step_s"], "admission_s": sat["admin_s"],
                      "admission_parts_s": {"load": sat["load_s"], "result": sat["result_s"], "free": sat["free_s"]},
+                     "host_work_overlapped_s": sat["host_overlapped_s"], "overlap": sat["overlap"],
                      "chunks": sat["chunks"]},
         "latency": {"arrival_rate_per_s": rate, "load": args.serve_load,
                     "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),

@@ -206,8 +206,10 @@ class GpuSim:
     def _body(self, gb):
         return self.current[gb]
 
-    def load_sim(self, si, w):
-        """Reconfigure slot si for workload w (within the slot's creation sizes) and upload it."""
+    def load_sim(self, si, w, preps=None):
+        """Reconfigure slot si for workload w (within the slot's creation sizes) and upload it.
+        preps: upload_arrays() of w's bodies, made beforehand (host work done off the GPU's
+        critical path)."""
         keep = []
         d = _sim_desc(w, keep)
         _check(self.lib.mpm_gpu_load_sim(self.h, si, C.byref(d)))
@@ -215,7 +217,7 @@ class GpuSim:
         self.grid_N[si] = tuple(w.N)
         for bi, b in enumerate(w.bodies):
             self.current[b0 + bi] = b
-            self.upload(b0 + bi, b)
+            self.upload(b0 + bi, b, prep=None if preps is None else preps[bi])
 
     def clear_sim(self, si):
         _check(self.lib.mpm_gpu_clear_sim(self.h, si))
@@ -232,15 +234,21 @@ class GpuSim:
         _check(self.lib.mpm_gpu_get_option(self.h, key, C.byref(v)))
         return v.value
 
-    def upload(self, gb, body, T=None):
-        dim = self.dim
-        n = body.n
+    @staticmethod
+    def upload_arrays(body, dim):
+        """The host arrays mpm_gpu_upload takes for `body` (component planes, extents):
+        numpy work only, so it can run while the GPU steps other simulations."""
         x = np.ascontiguousarray(np.asarray(body.x, np.float64)[:, :dim].T)
         v = np.ascontiguousarray(np.asarray(body.xt, np.float64)[:, :dim].T)
         V = np.ascontiguousarray(body.v, np.float64)
         m = np.ascontiguousarray(body.m, np.float64)
         ext = np.ascontiguousarray(extent_of(V, dim))
         act = np.ascontiguousarray(body.active, np.int32)
+        return x, v, V, m, ext, act
+
+    def upload(self, gb, body, T=None, prep=None):
+        n = body.n
+        x, v, V, m, ext, act = prep if prep is not None else self.upload_arrays(body, self.dim)
         Tp = None
         if T is not None:
             Tp = np.ascontiguousarray(np.asarray(T, np.float64).reshape(6, n))
@@ -252,19 +260,30 @@ class GpuSim:
     def step(self, dt, nsteps=1):
         _check(self.lib.mpm_gpu_step(self.h, float(dt), int(nsteps)))
 
-    def points(self, gb):
-        """State of global body gb in original order: x, xt (n, dim); T (n, 6); v (n,); active."""
+    def points_planes(self, gb):
+        """State of global body gb in original order, as downloaded: x, xt (dim, n) and
+        T (6, n) component planes; v (n,); active (n,)."""
         n = self._body(gb).n
         dim = self.dim
-        x = np.zeros((dim, n))
-        v = np.zeros((dim, n))
-        T = np.zeros((6, n))
-        V = np.zeros(n)
-        a = np.zeros(n, np.int32)
+        x = np.empty((dim, n))
+        v = np.empty((dim, n))
+        T = np.empty((6, n))
+        V = np.empty(n)
+        a = np.empty(n, np.int32)
         _check(self.lib.mpm_gpu_download(self.h, gb, x.ctypes.data_as(_dp), v.ctypes.data_as(_dp),
                                          T.ctypes.data_as(_dp), V.ctypes.data_as(_dp),
                                          a.ctypes.data_as(C.POINTER(C.c_int32))))
-        return {"x": x.T.copy(), "xt": v.T.copy(), "T6": T.T.copy(), "v": V, "active": a}
+        return {"x": x, "xt": v, "T6": T, "v": V, "active": a}
+
+    @staticmethod
+    def rows(planes):
+        """points_planes() output in the points() layout (one row per point)."""
+        return {"x": planes["x"].T.copy(), "xt": planes["xt"].T.copy(), "T6": planes["T6"].T.copy(),
+                "v": planes["v"], "active": planes["active"]}
+
+    def points(self, gb):
+        """State of global body gb in original order: x, xt (n, dim); T (n, 6); v (n,); active."""
+        return self.rows(self.points_planes(gb))
 
     def aux(self, gb):
         n = self._body(gb).n

@@ -155,8 +155,8 @@ class SlotPool:
     def request_of(self, s):
         return self.req[s]
 
-    def load(self, s, r, w, steps):
-        self.sim.load_sim(s, w)
+    def load(self, s, r, w, steps, prep=None):
+        self.sim.load_sim(s, w, preps=prep)
         self.req[s], self.left[s], self.dt[s], self.loaded[s] = r, steps, w.dt, w
 
     def step(self, k):
@@ -165,8 +165,21 @@ class SlotPool:
             self.left[s] -= k
 
     def result(self, s):
+        return self.finish(self.fetch(s))
+
+    # -- host work split from the device work, so run_stream can overlap it with stepping --
+    def prepare(self, w):
+        """Host arrays of w's uploads (no GPU call)."""
+        return [GpuSim.upload_arrays(b, self.dim) for b in w.bodies]
+
+    def fetch(self, s):
+        """Download slot s's bodies as component planes (GPU calls)."""
         b0 = self.sim.body0[s]
-        return [self.sim.points(b0 + b) for b in range(len(self.loaded[s].bodies))]
+        return [self.sim.points_planes(b0 + b) for b in range(len(self.loaded[s].bodies))]
+
+    def finish(self, fetched):
+        """fetch() output in the result layout (no GPU call)."""
+        return [GpuSim.rows(p) for p in fetched]
 
     def free(self, s):
         self.sim.clear_sim(s)
@@ -193,15 +206,24 @@ def check_result(w, bodies):
 
 def run_stream(pool, requests: Sequence[Tuple[W.Workload, int]], arrivals: Sequence[float], chunk_max=64,
                keep_results=False, log=None, cancel_at: Optional[Dict[int, float]] = None, clock=None,
-               sleep=None):
+               sleep=None, overlap: Optional[bool] = None, prep_ahead=4):
     """Serve `requests` arriving at `arrivals` (seconds from start, sorted). Returns per-
     request (arrival, start, finish) times and status, the results if kept, and the
     makespan. Admission is first come, first served over the slots a request fits,
     smallest slot first: a free slot goes to the earliest waiting request that fits it, so
-    a large request waiting for a large slot gets it before any later request."""
+    a large request waiting for a large slot gets it before any later request.
+
+    overlap: each chunk of steps runs on a worker thread while this thread does the host work
+    around it: the result layout and non-finite check of the requests that finished at the
+    last chunk boundary (downloaded there, their slots already free) and the upload arrays of
+    the first `prep_ahead` waiting requests. Only calls into the executor's device (load,
+    fetch, free) stay between chunks. Needs an executor with prepare, fetch and finish; by
+    default on when it has them."""
     clock = clock or time.perf_counter
     sleep = sleep or time.sleep
     cancel_at = dict(cancel_at or {})
+    if overlap is None:
+        overlap = all(hasattr(pool, a) for a in ("prepare", "fetch", "finish"))
     n = len(requests)
     t0 = clock()
     queue: List[int] = []
@@ -213,7 +235,10 @@ def run_stream(pool, requests: Sequence[Tuple[W.Workload, int]], arrivals: Seque
     done = 0
     t_step = t_admin = 0.0
     t_load = t_result = t_free = 0.0  # parts of t_admin
+    t_host = 0.0  # host work done while a chunk stepped (overlap)
     chunks = 0
+    prepared: Dict[int, list] = {}  # request -> its upload arrays (overlap)
+    fetched: List[Tuple[int, list]] = []  # finished requests downloaded, not yet checked
 
     def close(r, st):
         nonlocal done
@@ -221,87 +246,129 @@ def run_stream(pool, requests: Sequence[Tuple[W.Workload, int]], arrivals: Seque
         finish[r] = clock() - t0
         done += 1
 
-    while done < n:
-        now = clock() - t0
-        while nxt < n and arrivals[nxt] <= now:
-            w = requests[nxt][0]
-            if pool.fits_any(w):
-                queue.append(nxt)
-            else:
-                close(nxt, "refused")
-                if log:
-                    log(f"request {nxt} refused: larger than every slot")
-            nxt += 1
-        ta = clock()
-        # cancellations due: waiting requests leave the queue, running ones free their slot
-        for r, tc in list(cancel_at.items()):
-            if tc > now or status[r] != "pending":
+    def complete(r, res):
+        why = check_result(requests[r][0], res)
+        if keep_results:
+            results[r] = res
+        close(r, "done" if why is None else f"failed: {why}")
+        if log:
+            log(f"request {r} {status[r]}: {requests[r][0].n_points} points x {requests[r][1]} steps, "
+                f"latency {finish[r] - arrivals[r]:.3f} s")
+
+    def finish_fetched():
+        for r, raw in fetched:
+            complete(r, pool.finish(raw))
+        fetched.clear()
+
+    worker = None
+    if overlap:
+        from concurrent.futures import ThreadPoolExecutor
+        worker = ThreadPoolExecutor(max_workers=1)
+    try:
+        while done < n:
+            now = clock() - t0
+            while nxt < n and arrivals[nxt] <= now:
+                w = requests[nxt][0]
+                if pool.fits_any(w):
+                    queue.append(nxt)
+                else:
+                    close(nxt, "refused")
+                    if log:
+                        log(f"request {nxt} refused: larger than every slot")
+                nxt += 1
+            ta = clock()
+            # cancellations due: waiting requests leave the queue, running ones free their slot
+            for r, tc in list(cancel_at.items()):
+                if tc > now or status[r] != "pending":
+                    continue
+                if r in queue:
+                    queue.remove(r)
+                    prepared.pop(r, None)
+                    close(r, "cancelled")
+                    del cancel_at[r]
+                else:
+                    for s in pool.active_slots():
+                        if pool.request_of(s) == r:
+                            pool.free(s)
+                            close(r, "cancelled")
+                            del cancel_at[r]
+                            break
+            # admission: in queue order, each request into the smallest free slot it fits;
+            # one that fits none waits, and later ones may take what is left
+            free = sorted(pool.free_slots(), key=pool.capacity)
+            for r in list(queue):
+                if not free:
+                    break
+                w, steps = requests[r]
+                cand = [s for s in free if pool.fits(s, w)]
+                if cand:
+                    tl = clock()
+                    if overlap:
+                        pool.load(cand[0], r, w, steps, prep=prepared.pop(r, None))
+                    else:
+                        pool.load(cand[0], r, w, steps)
+                    t_load += clock() - tl
+                    start[r] = clock() - t0
+                    queue.remove(r)
+                    free.remove(cand[0])
+            t_admin += clock() - ta
+            active = pool.active_slots()
+            if not active:
+                if fetched:  # nothing steps: check the last results now
+                    ta = clock()
+                    finish_fetched()
+                    t_result += clock() - ta
+                    t_admin += clock() - ta
+                    continue
+                if nxt < n:
+                    wait = arrivals[nxt] - (clock() - t0)
+                    nexts = [tc for r, tc in cancel_at.items() if status[r] == "pending"]
+                    if nexts:
+                        wait = min(wait, min(nexts) - (clock() - t0))
+                    if wait > 0:
+                        sleep(wait)
+                elif queue:
+                    raise RuntimeError("requests wait for slots while none are in use")
                 continue
-            if r in queue:
-                queue.remove(r)
-                close(r, "cancelled")
-                del cancel_at[r]
+            k = min(chunk_max, min(pool.remaining(s) for s in active))
+            ts = clock()
+            if overlap:
+                job = worker.submit(pool.step, k)
+                th = clock()
+                finish_fetched()
+                for r in queue[:prep_ahead]:
+                    if r not in prepared:
+                        prepared[r] = pool.prepare(requests[r][0])
+                t_host += clock() - th
+                job.result()
             else:
-                for s in pool.active_slots():
-                    if pool.request_of(s) == r:
-                        pool.free(s)
-                        close(r, "cancelled")
-                        del cancel_at[r]
-                        break
-        # admission: in queue order, each request into the smallest free slot it fits;
-        # one that fits none waits, and later ones may take what is left
-        free = sorted(pool.free_slots(), key=pool.capacity)
-        for r in list(queue):
-            if not free:
-                break
-            w, steps = requests[r]
-            cand = [s for s in free if pool.fits(s, w)]
-            if cand:
-                tl = clock()
-                pool.load(cand[0], r, w, steps)
-                t_load += clock() - tl
-                start[r] = clock() - t0
-                queue.remove(r)
-                free.remove(cand[0])
-        t_admin += clock() - ta
-        active = pool.active_slots()
-        if not active:
-            if nxt < n:
-                wait = arrivals[nxt] - (clock() - t0)
-                nexts = [tc for r, tc in cancel_at.items() if status[r] == "pending"]
-                if nexts:
-                    wait = min(wait, min(nexts) - (clock() - t0))
-                if wait > 0:
-                    sleep(wait)
-            elif queue:
-                raise RuntimeError("requests wait for slots while none are in use")
-            continue
-        k = min(chunk_max, min(pool.remaining(s) for s in active))
-        ts = clock()
-        pool.step(k)
-        t_step += clock() - ts
-        chunks += 1
-        ta = clock()
-        for s in active:
-            if pool.remaining(s) == 0:
-                r = pool.request_of(s)
-                tr = clock()
-                res = pool.result(s)
-                why = check_result(requests[r][0], res)
-                t_result += clock() - tr
-                if keep_results:
-                    results[r] = res
-                tf = clock()
-                pool.free(s)
-                t_free += clock() - tf
-                close(r, "done" if why is None else f"failed: {why}")
-                if log:
-                    log(f"request {r} {status[r]}: {requests[r][0].n_points} points x {requests[r][1]} steps, "
-                        f"latency {finish[r] - arrivals[r]:.3f} s")
-        t_admin += clock() - ta
+                pool.step(k)
+            t_step += clock() - ts
+            chunks += 1
+            ta = clock()
+            for s in active:
+                if pool.remaining(s) == 0:
+                    r = pool.request_of(s)
+                    tr = clock()
+                    if overlap:
+                        fetched.append((r, pool.fetch(s)))
+                        t_result += clock() - tr
+                    else:
+                        res = pool.result(s)
+                        t_result += clock() - tr
+                    tf = clock()
+                    pool.free(s)
+                    t_free += clock() - tf
+                    if not overlap:
+                        complete(r, res)
+            t_admin += clock() - ta
+    finally:
+        if worker is not None:
+            worker.shutdown(wait=True)
     return {"arrival": list(arrivals), "start": start, "finish": finish, "status": status, "results": results,
             "makespan": clock() - t0, "step_s": t_step, "admin_s": t_admin, "chunks": chunks,
-            "load_s": t_load, "result_s": t_result, "free_s": t_free}
+            "load_s": t_load, "result_s": t_result, "free_s": t_free, "host_overlapped_s": t_host,
+            "overlap": bool(overlap)}
 
 
 def poisson_arrivals(n, rate, seed=0xC0FFEE + 5):

@@ -108,14 +108,17 @@ def test_autotune_sort_interval_keeps_results():
         assert ex <= 1e-9 and ev <= 1e-8
 
 
-def test_served_results_match_alone():
+@pytest.mark.parametrize("overlap", [False, True])
+def test_served_results_match_alone(overlap):
     """Requests admitted into a 3-slot pool as slots free up (one slot reused twice), with
-    others running: every result equals the request run alone."""
+    others running: every result equals the request run alone; with the host work inline,
+    and overlapped with the stepping (chunks stepped on a worker thread)."""
     reqs = [(w, s // 20) for w, s in W.serve_requests(40, seed=21) if w.n_points < 60000][:5]
     assert len(reqs) == 5
     pool = S.SlotPool(3, dim=3, N_cap=128, body_caps=(60000, 60000), precision=MPM_PREC_FP32)
-    out = S.run_stream(pool, reqs, [0.0] * len(reqs), chunk_max=48, keep_results=True)
+    out = S.run_stream(pool, reqs, [0.0] * len(reqs), chunk_max=48, keep_results=True, overlap=overlap)
     pool.close()
+    assert out["overlap"] == overlap and out["status"] == ["done"] * len(reqs)
     for r, (w, steps) in enumerate(reqs):
         alone = S.run_alone(w, steps)
         for b in range(len(w.bodies)):

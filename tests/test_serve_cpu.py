@@ -2,8 +2,9 @@
 first come, first served over the slots a request fits, smallest slot first, no starvation
 of large requests by small ones, refusal of requests no slot can hold, cancellation of
 waiting and running requests, and non-finite results reported as failures without
-disturbing the others."""
+disturbing the others; each with the host work done inline and overlapped with stepping."""
 import numpy as np
+import pytest
 
 from mpm_3d_amd import serve as S
 from mpm_3d_amd import workloads as W
@@ -84,6 +85,37 @@ class MockPool:
         self.left[s] = 0
 
 
+class MockPoolOverlap(MockPool):
+    """MockPool with the host/device split of SlotPool (prepare, fetch, finish), so the
+    serving loop steps each chunk on a worker thread and does the host work meanwhile."""
+
+    def __init__(self, *a, **k):
+        super().__init__(*a, **k)
+        self.prepared = []   # requests whose upload arrays were made
+        self.with_prep = []  # requests loaded with prepared arrays
+
+    def load(self, s, r, w, steps, prep=None):
+        super().load(s, r, w, steps)
+        if prep is not None:
+            assert prep == ("prep", w.name)
+            self.with_prep.append(r)
+
+    def prepare(self, w):
+        self.prepared.append(w.name)
+        return ("prep", w.name)
+
+    def fetch(self, s):
+        return ("fetched", self.result(s))
+
+    def finish(self, fetched):
+        assert fetched[0] == "fetched"
+        return fetched[1]
+
+
+def pool_of(overlap, caps, clk):
+    return (MockPoolOverlap if overlap else MockPool)(caps, clk)
+
+
 def req(name, N, npts, nbody=1):
     bodies = [W.Body(f"b{i}", np.ones(npts), np.ones(npts), np.zeros((npts, 3)), np.zeros((npts, 3)),
                      np.ones(npts, np.int32), 1e6, 0.3, True) for i in range(nbody)]
@@ -95,12 +127,16 @@ SMALL = (64, (300, 300))
 BIG = (128, (2000, 1000))
 
 
-def test_fifo_smallest_slot_and_reuse():
+@pytest.mark.parametrize("overlap", [False, True])
+def test_fifo_smallest_slot_and_reuse(overlap):
     clk = Clock()
-    pool = MockPool([BIG, SMALL, SMALL], clk)
+    pool = pool_of(overlap, [BIG, SMALL, SMALL], clk)
     reqs = [(req(f"r{i}", 40, 100), 10 * (i + 1)) for i in range(6)]
     out = S.run_stream(pool, reqs, [0.0] * 6, chunk_max=8, clock=clk, sleep=clk.sleep)
+    assert out["overlap"] == overlap
     assert out["status"] == ["done"] * 6
+    if overlap:  # the three that waited were loaded with upload arrays made during stepping
+        assert sorted(pool.with_prep) == [3, 4, 5]
     # the two small slots take the first two; the big slot is used only when they are busy
     first = {r: s for s, r, _ in pool.loads[:3]}
     assert first[0] in (1, 2) and first[1] in (1, 2) and first[2] == 0
@@ -111,9 +147,10 @@ def test_fifo_smallest_slot_and_reuse():
         assert out["finish"][r] >= out["start"][r] + st * pool.step_s - 1e-12
 
 
-def test_large_request_not_starved_by_small_ones():
+@pytest.mark.parametrize("overlap", [False, True])
+def test_large_request_not_starved_by_small_ones(overlap):
     clk = Clock()
-    pool = MockPool([BIG, SMALL], clk)
+    pool = pool_of(overlap, [BIG, SMALL], clk)
     # r0 occupies the big slot; r1 (big) waits for it; r2.. (small) must not take the big
     # slot when it frees, although they could use it
     reqs = [(req("r0", 100, 1500), 20), (req("r1", 100, 1500), 20)]
@@ -124,9 +161,10 @@ def test_large_request_not_starved_by_small_ones():
     assert big_loads[:2] == [0, 1]
 
 
-def test_refused_cancelled_and_failed():
+@pytest.mark.parametrize("overlap", [False, True])
+def test_refused_cancelled_and_failed(overlap):
     clk = Clock()
-    pool = MockPool([SMALL, SMALL], clk)
+    pool = pool_of(overlap, [SMALL, SMALL], clk)
     reqs = [
         (req("ok0", 40, 100), 40),
         (req("too_big_grid", 96, 100), 10),   # N above every slot
@@ -151,9 +189,10 @@ def test_refused_cancelled_and_failed():
     assert not any(r in (1, 2, 4) for _, r, _ in pool.loads)
 
 
-def test_waits_for_arrivals():
+@pytest.mark.parametrize("overlap", [False, True])
+def test_waits_for_arrivals(overlap):
     clk = Clock()
-    pool = MockPool([SMALL], clk)
+    pool = pool_of(overlap, [SMALL], clk)
     reqs = [(req(f"r{i}", 40, 100), 10) for i in range(3)]
     arrivals = [0.5, 1.0, 1.01]
     out = S.run_stream(pool, reqs, arrivals, chunk_max=10, clock=clk, sleep=clk.sleep)
