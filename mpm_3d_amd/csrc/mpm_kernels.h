@@ -360,9 +360,9 @@ void k_particles(KArgs A, int want_L) {
     constexpr int NOUT = GF<DIM>::NOUT;
     constexpr int NZ = DIM == 3 ? 3 : 1;
     constexpr int PBLKS = Cfg<DIM>::PBLKS;
-    // P2G accumulator patch, FP64 in both forms. (Keeping the precision form's force partial
-    // sums in FP32 was measured 5x slower on gfx950: FP32 LDS atomics (ds_add_f32) are much
-    // slower than ds_add_f64 there.)
+    // P2G accumulator patch, FP64 in both forms. (Keeping the precision form's three force
+    // planes in FP32 with ds_add_f32 took the 16M-column P2G from 1.25 to 3.43 ms on gfx950,
+    // with 22x the LDS issue waits per wavefront at the same LDS instruction count.)
     __shared__ double lacc[DO_P2G ? NACC * PSTR : 1];
     __shared__ GO gpat[DO_G2P ? NOUT * PSTR : 1];
     __shared__ int bb[2 * DIM];
