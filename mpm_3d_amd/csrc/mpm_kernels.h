@@ -264,20 +264,29 @@ __device__ __forceinline__ void side_done(unsigned *rare_n, int k) {
 #define MPM_WPE_G2P32 4
 #endif
 #ifndef MPM_WPE_P2G32
-#define MPM_WPE_P2G32 2
+#define MPM_WPE_P2G32 4
 #endif
 #ifndef MPM_PF_G2P
 #define MPM_PF_G2P 1
 #endif
-#ifndef MPM_PF_P2G
-#define MPM_PF_P2G 0
+#ifndef MPM_PF_P2G32
+#define MPM_PF_P2G32 1
 #endif
-// Lane order of the P2G pass inside a wavefront: 0 slot order; 1 and 2 place the point of
-// in-block dual cell (x, y, z) on lane x + 4y + 16z (1) or x + 4z + 16y (2) (2D: x + 8y,
-// y + 8x): one point of each dual cell there, any further ones on the lanes left free, so
-// that the lanes served together by the LDS hit distinct banks.
-#ifndef MPM_P2G_ORDER
-#define MPM_P2G_ORDER 2
+#ifndef MPM_PF_P2G64
+#define MPM_PF_P2G64 0
+#endif
+// Lane order of the P2G pass inside a wavefront, per form: 0 slot order; 1 and 2 place the
+// point of in-block dual cell (x, y, z) on lane x + 4y + 16z (1) or x + 4z + 16y (2) (2D:
+// x + 8y, y + 8x): one point of each dual cell there, any further ones on the lanes left
+// free, so that the lanes served together by the LDS hit distinct banks. The order costs a
+// second load trip (the dual cells first, then each lane's point). Measured on the 16M column:
+// FP64 P2G 1.33 ms with order 2 against 1.39 in slot order; FP32 P2G 1.26 ms with order 2
+// against 1.22 in slot order and 1.20 in slot order with the next point prefetched.
+#ifndef MPM_P2G_ORDER32
+#define MPM_P2G_ORDER32 0
+#endif
+#ifndef MPM_P2G_ORDER64
+#define MPM_P2G_ORDER64 2
 #endif
 #ifndef MPM_DYN_BLOCKS
 #define MPM_DYN_BLOCKS 1
@@ -335,7 +344,7 @@ template <bool DO_G2P, bool DO_P2G, bool FP32> struct PassTune {
     static constexpr int waves = (DO_G2P && DO_P2G) ? 2  // the fused pass
                                  : DO_G2P ? (FP32 ? MPM_WPE_G2P32 : MPM_WPE_G2P64)
                                           : (FP32 ? MPM_WPE_P2G32 : MPM_WPE_P2G64);
-    static constexpr bool prefetch = DO_G2P ? MPM_PF_G2P != 0 : MPM_PF_P2G != 0;
+    static constexpr bool prefetch = DO_G2P ? MPM_PF_G2P != 0 : (FP32 ? MPM_PF_P2G32 != 0 : MPM_PF_P2G64 != 0);
 };
 
 // Particle pass over one particle block per workgroup (persistent over the block list).
@@ -348,7 +357,7 @@ template <bool DO_G2P, bool DO_P2G, bool FP32> struct PassTune {
 template <int DIM, bool DO_G2P, bool DO_P2G, bool NRM, class PT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PassTune<DO_G2P, DO_P2G, PT::FP32>::waves)))
 void k_particles(KArgs A, int want_L) {
-    constexpr int ORDER = (!DO_G2P && DO_P2G) ? MPM_P2G_ORDER : 0;
+    constexpr int ORDER = (!DO_G2P && DO_P2G) ? (PT::FP32 ? MPM_P2G_ORDER32 : MPM_P2G_ORDER64) : 0;
     constexpr bool DYN = MPM_DYN_BLOCKS != 0 && dyn_blocks<DIM, DO_G2P, DO_P2G, PT::FP32>();
     constexpr bool SIDE = side_pass<DO_G2P, DO_P2G, PT::FP32>();
     constexpr bool PREFETCH = PassTune<DO_G2P, DO_P2G, PT::FP32>::prefetch;
