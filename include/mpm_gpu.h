@@ -207,9 +207,9 @@ int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
  *                          a captured HIP graph (split passes, timing off); 0 (default)
  *   MPM_OPT_AUTOTUNE       1: the next step call first times candidate launch shapes
  *                          (persistent particle-pass grids, grid-update grid) and then, when
- *                          it has 57 steps to spare, sort intervals 8, 16 and 32 (and 64 with
- *                          121 to spare), on its own first steps, and keeps the fastest; those
- *                          steps count. Reads 2 once tuned
+ *                          it has 57 steps to spare, sort intervals 8, 16 and 32, on its own
+ *                          first steps, and keeps the fastest; those steps count. Reads 2
+ *                          once tuned
  *   MPM_OPT_PASS_GRID      workgroups of the persistent G2P and fused particle passes
  *                          (default 1024); setting it sets MPM_OPT_P2G_GRID too
  *   MPM_OPT_P2G_GRID       workgroups of the persistent P2G pass (default: as many as the

@@ -89,8 +89,8 @@ def test_autotune_keeps_results():
 
 
 def test_autotune_sort_interval_keeps_results():
-    """A call long enough to tune the sort interval too (8, 16, 32 or 64 steps per binning),
-    with graph replay after it: the same state as a handle that never tuned, to rounding."""
+    """A call long enough to tune the sort interval too (8, 16 or 32 steps per binning), with
+    graph replay after it: the same state as a handle that never tuned, to rounding."""
     ws = _small_ensemble()
     a = GpuSim(ws, precision=MPM_PREC_FP64)
     b = GpuSim(ws, precision=MPM_PREC_FP64)
@@ -100,7 +100,7 @@ def test_autotune_sort_interval_keeps_results():
     a.step_batch(dts, 150)
     b.step_batch(dts, 150)
     assert b.get_option(MPM_OPT_AUTOTUNE) == 2
-    assert b.get_option(MPM_OPT_SORT_INTERVAL) in (8, 16, 32, 64)
+    assert b.get_option(MPM_OPT_SORT_INTERVAL) in (8, 16, 32)
     b.step_batch(dts, 70)
     a.step_batch(dts, 70)
     for i, w in enumerate(ws):
