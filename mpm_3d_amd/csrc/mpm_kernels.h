@@ -282,6 +282,10 @@ __device__ __forceinline__ void side_done(unsigned *rare_n, int k) {
 // second load trip (the dual cells first, then each lane's point). Measured on the 16M column:
 // FP64 P2G 1.33 ms with order 2 against 1.39 in slot order; FP32 P2G 1.26 ms with order 2
 // against 1.22 in slot order and 1.20 in slot order with the next point prefetched.
+// G2P pass in 3D: load the next particle block's node patch by LDS DMA during this block
+#ifndef MPM_G2P_ASYNC_PATCH
+#define MPM_G2P_ASYNC_PATCH 1
+#endif
 #ifndef MPM_P2G_ORDER32
 #define MPM_P2G_ORDER32 0
 #endif
@@ -374,6 +378,11 @@ void k_particles(KArgs A, int want_L) {
     // with 22x the LDS issue waits per wavefront at the same LDS instruction count.)
     __shared__ double lacc[DO_P2G ? NACC * PSTR : 1];
     __shared__ GO gpat[DO_G2P ? NOUT * PSTR : 1];
+    // G2P pass (3D): the next particle block's node patch, loaded by LDS DMA while this
+    // block's points run, in x-pair order [by, bz pair][field][2 node blocks x 64 nodes]
+    constexpr bool ASYNC = MPM_G2P_ASYNC_PATCH != 0 && DO_G2P && !DO_P2G && DIM == 3;
+    __shared__ GO graw[ASYNC ? 4 * NOUT * 128 : 1];
+    bool pre = false;  // graw holds this block's patch (wavefront-uniform)
     __shared__ int bb[2 * DIM];
     __shared__ int wslot[ORDER ? 512 : 1];  // per wavefront: the lane holding each key; left-over points
     const int tid = threadIdx.x;
@@ -419,7 +428,24 @@ void k_particles(KArgs A, int want_L) {
                 bb[DIM + tid] = -0x7fffffff;
             }
         }
-        if (DO_G2P) {
+        if (ASYNC && pre) {
+            // the patch arrived by DMA during the previous block: lay it out like the
+            // staging below (zeros outside the domain)
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            for (int i = tid; i < PBLKS * 64; i += blockDim.x) {
+                int nijk[DIM];
+                bool inp, in;
+                (void)patch_node<DIM>(s, org, i, nijk, inp, in);
+                if (inp) {
+                    const int li = patch_index<DIM>(nijk, org);
+                    const int b = i >> 6;
+                    const int src = (b >> 1) * NOUT * 128 + (b & 1) * 64 + (i & 63);
+#pragma unroll
+                    for (int f = 0; f < NOUT; f++) gpat[f * PSTR + li] = in ? graw[src + f * 128] : (GO)0;
+                }
+            }
+        } else if (DO_G2P) {
             // stage the block's node patch. The patch starts on a node-block boundary, so it
             // is read as whole 64-node blocks: one node per thread, all its fields at once.
             for (int i = tid; i < PBLKS * 64; i += blockDim.x) {
@@ -457,6 +483,53 @@ void k_particles(KArgs A, int want_L) {
             PT::template load_raw<DIM>(A, start + tid, qn, need_vol);
         }
         const int lane = tid & 63;
+        if (ASYNC) {
+            // the workgroup's next block: DMA its patch into graw (after the prefetch above,
+            // whose wait must not include these loads). Each instruction moves the two
+            // x-adjacent node blocks (contiguous in storage) of one field, 16 bytes a lane;
+            // node blocks outside the simulation's block grid are not read.
+            // (the block list and descriptors are read through the constant address space:
+            // scalar loads, whose wait does not take in the vector loads in flight)
+            pre = false;
+            const int en = dyn ? uniform(e_sh[epar ^ 1]) : e + (int)gridDim.x;
+            if (en < nent) {
+                using CPB = const __attribute__((address_space(4))) PBlk;
+                using CBD = const __attribute__((address_space(4))) BodyDev;
+                using CSD = const __attribute__((address_space(4))) SimDev;
+                CPB &ne = ((CPB *)A.plist)[en];
+                CBD &B2 = ((CBD *)A.bodies)[ne.body];
+                CSD &s2 = ((CSD *)A.sims)[B2.sim];
+                int org2[DIM];
+                {  // patch_origin, on the scalar copy
+                    long long lk = ne.key - B2.pblk_off;
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        const int pb = (int)(lk % s2.PNB[d]);
+                        lk /= s2.PNB[d];
+                        org2[d] = pb * Cfg<DIM>::PB - 2;
+                    }
+                }
+                int bc[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; d++) bc[d] = (org2[d] + NSHIFT) / Cfg<DIM>::NB;
+                constexpr int PER = 16 / (int)sizeof(GO);  // values per lane
+                const int half = 64 / PER;                 // lanes per node block
+                const int xb = bc[0] + (lane >= half ? 1 : 0);
+                const GO *src0 = sizeof(GO) == 8 ? reinterpret_cast<const GO *>(A.out) : reinterpret_cast<const GO *>(A.out32);
+                for (int j = tid >> 6; j < 4 * NOUT; j += (int)(blockDim.x >> 6)) {
+                    const int pair = j / NOUT, f = j - pair * NOUT;
+                    const int by = bc[1] + (pair & 1), bz = bc[2] + (pair >> 1);
+                    if (lane < 2 * half && xb < s2.NBn[0] && by < s2.NBn[1] && bz < s2.NBn[2]) {
+                        const long long blk = bc[0] + (long long)s2.NBn[0] * (by + (long long)s2.NBn[1] * bz);
+                        const GO *g = src0 + (long long)f * A.nt + B2.grid_base + blk * 64 + (long long)lane * PER;
+                        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)g,
+                                                         (__attribute__((address_space(3))) void *)(graw + j * 128),
+                                                         16, 0, 0);
+                    }
+                }
+                pre = true;
+            }
+        }
 #pragma unroll 1
         for (int kw = ORDER ? tid - lane : tid; kw < count; kw += blockDim.x) {  // ORDER: wavefront-uniform
             int k = ORDER ? kw + lane : kw;
@@ -777,6 +850,7 @@ void k_particles(KArgs A, int want_L) {
             e += gridDim.x;
         }
     }
+    if (ASYNC) __builtin_amdgcn_s_waitcnt(0);  // no LDS DMA in flight when the workgroup ends
     if (dyn) {
         // the last workgroup to finish clears the counter for the next pass: every workgroup
         // read its final (out of range) index before it signs off here
