@@ -378,10 +378,11 @@ void k_particles(KArgs A, int want_L) {
     // with 22x the LDS issue waits per wavefront at the same LDS instruction count.)
     __shared__ double lacc[DO_P2G ? NACC * PSTR : 1];
     __shared__ GO gpat[DO_G2P ? NOUT * PSTR : 1];
-    // G2P pass (3D): the next particle block's node patch, loaded by LDS DMA while this
-    // block's points run, in x-pair order [by, bz pair][field][2 node blocks x 64 nodes]
-    constexpr bool ASYNC = MPM_G2P_ASYNC_PATCH != 0 && DO_G2P && !DO_P2G && DIM == 3;
-    __shared__ GO graw[ASYNC ? 4 * NOUT * 128 : 1];
+    // G2P pass: the next particle block's node patch, loaded by LDS DMA while this block's
+    // points run, in x-pair order [y (, z) pair][field][2 node blocks x 64 nodes]
+    constexpr bool ASYNC = MPM_G2P_ASYNC_PATCH != 0 && DO_G2P && !DO_P2G;
+    constexpr int NPAIR = PBLKS / 2;
+    __shared__ GO graw[ASYNC ? NPAIR * NOUT * 128 : 1];
     bool pre = false;  // graw holds this block's patch (wavefront-uniform)
     __shared__ int bb[2 * DIM];
     __shared__ int wslot[ORDER ? 512 : 1];  // per wavefront: the lane holding each key; left-over points
@@ -516,10 +517,10 @@ void k_particles(KArgs A, int want_L) {
                 const int half = 64 / PER;                 // lanes per node block
                 const int xb = bc[0] + (lane >= half ? 1 : 0);
                 const GO *src0 = sizeof(GO) == 8 ? reinterpret_cast<const GO *>(A.out) : reinterpret_cast<const GO *>(A.out32);
-                for (int j = tid >> 6; j < 4 * NOUT; j += (int)(blockDim.x >> 6)) {
+                for (int j = tid >> 6; j < NPAIR * NOUT; j += (int)(blockDim.x >> 6)) {
                     const int pair = j / NOUT, f = j - pair * NOUT;
-                    const int by = bc[1] + (pair & 1), bz = bc[2] + (pair >> 1);
-                    if (lane < 2 * half && xb < s2.NBn[0] && by < s2.NBn[1] && bz < s2.NBn[2]) {
+                    const int by = bc[1] + (pair & 1), bz = DIM == 3 ? bc[DIM - 1] + (pair >> 1) : 0;
+                    if (lane < 2 * half && xb < s2.NBn[0] && by < s2.NBn[1] && (DIM == 2 || bz < s2.NBn[DIM - 1])) {
                         const long long blk = bc[0] + (long long)s2.NBn[0] * (by + (long long)s2.NBn[1] * bz);
                         const GO *g = src0 + (long long)f * A.nt + B2.grid_base + blk * 64 + (long long)lane * PER;
                         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)g,
