@@ -384,6 +384,7 @@ void k_particles(KArgs A, int want_L) {
     constexpr int NPAIR = PBLKS / 2;
     __shared__ GO graw[ASYNC ? NPAIR * NOUT * 128 : 1];
     bool pre = false;  // graw holds this block's patch (wavefront-uniform)
+    bool first_blk = true;
     __shared__ int bb[2 * DIM];
     __shared__ int wslot[ORDER ? 512 : 1];  // per wavefront: the lane holding each key; left-over points
     const int tid = threadIdx.x;
@@ -423,7 +424,11 @@ void k_particles(KArgs A, int want_L) {
         int org[DIM];
         patch_origin<DIM>(s, uniform(ent.key) - B.pblk_off, org);
         if (DO_P2G) {
-            for (int i = tid; i < NACC * PSTR; i += blockDim.x) lacc[i] = 0.0;
+            // lacc is all zero here: cleared before the first block, and the flush clears
+            // what a block wrote (every write lies inside the flushed hull)
+            if (first_blk)
+                for (int i = tid; i < NACC * PSTR; i += blockDim.x) lacc[i] = 0.0;
+            first_blk = false;
             if (tid < DIM) {
                 bb[tid] = 0x7fffffff;
                 bb[DIM + tid] = -0x7fffffff;
@@ -833,7 +838,10 @@ void k_particles(KArgs A, int want_L) {
                 for (int f = 0; f < NACC; f++) {
                     if (f >= 1 + 2 * DIM && !b1) continue;
                     const double v = lacc[f * PSTR + li];
-                    if (v != 0.0) unsafeAtomicAdd(&A.acc[f * A.nt + gi], v);
+                    if (v != 0.0) {
+                        unsafeAtomicAdd(&A.acc[f * A.nt + gi], v);
+                        lacc[f * PSTR + li] = 0.0;
+                    }
                 }
             }
             if (tid == 0 && total > 0) {
