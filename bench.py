@@ -283,7 +283,8 @@ def run_served(args, full_rate=None):
     counts = [int(c) for c in args.serve_slots.split(",")]
     classes = [S.SlotClass(c, N, caps) for c, (N, caps) in zip(counts, SERVE_CLASSES)]
     pu = sum(w.n_points * st for w, st in reqs)
-    pool = S.SlotPool(classes=classes, dim=3, precision=MPM_PREC_FP32, graphs=not args.no_graph)
+    pool = S.SlotPool(classes=classes, dim=3, precision=MPM_PREC_FP32, graphs=not args.no_graph,
+                      autotune=args.serve_autotune)
     sat = S.run_stream(pool, reqs, [0.0] * len(reqs), chunk_max=64)
     cap_solves = len(reqs) / sat["makespan"]
     rate = args.serve_load * cap_solves
@@ -356,6 +357,7 @@ def main():
     ap.add_argument("--serve-sample", type=int, default=4,
                     help="time every n-th served request alone for the one-at-a-time figure")
     ap.add_argument("--serve-load", type=float, default=0.75, help="arrival rate as a fraction of capacity")
+    ap.add_argument("--serve-autotune", action="store_true", help="served pool tunes its launch shapes")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 10 launches) used to calibrate counters")
     args = ap.parse_args()

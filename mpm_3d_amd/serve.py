@@ -32,8 +32,8 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from mpm_3d_amd import workloads as W
-from mpm_3d_amd.gpu import (MPM_OPT_GRAPH, MPM_OPT_KEEP_L, MPM_OPT_SORT_INTERVAL, MPM_PREC_FP32, GpuSim, load_library,
-                             _sim_desc)
+from mpm_3d_amd.gpu import (MPM_OPT_AUTOTUNE, MPM_OPT_GRAPH, MPM_OPT_KEEP_L, MPM_OPT_SORT_INTERVAL, MPM_PREC_FP32,
+                             GpuSim, load_library, _sim_desc)
 
 
 @dataclasses.dataclass
@@ -108,7 +108,7 @@ class SlotPool:
     the 512-simulation ensemble; a pool's mix changes as it runs, so it is not tuned here)."""
 
     def __init__(self, n_slots=None, dim=3, N_cap=128, body_caps=(2_000_000, 1_000_000), precision=MPM_PREC_FP32,
-                 graphs=True, classes=None, keep_L=False, sort_interval=32):
+                 graphs=True, classes=None, keep_L=False, sort_interval=32, autotune=False):
         if classes is None:
             classes = [SlotClass(int(n_slots), N_cap, tuple(body_caps))]
         self.dim = dim
@@ -125,6 +125,8 @@ class SlotPool:
         if not keep_L:
             self.sim.set_option(MPM_OPT_KEEP_L, 0)
         self.sim.set_option(MPM_OPT_SORT_INTERVAL, int(sort_interval))
+        if autotune:  # launch shapes, on the first chunk long enough (28 steps)
+            self.sim.set_option(MPM_OPT_AUTOTUNE, 1)
         if graphs:
             self.sim.set_option(MPM_OPT_GRAPH, 1)
         self.req = [None] * self.n_slots     # request index per slot
