@@ -27,18 +27,20 @@
 namespace mpm {
 
 // A node block's flag holds one bit per body of its simulation that scattered into it this
-// step (bit b: the sim's body b); the first mark appends the block to the list of parity par.
-__device__ __forceinline__ void mark_block(const KArgs &A, long long gblk, unsigned bit) {
-    if (atomicOr(&A.nflag[gblk], bit) == 0u) {
+// step (bit b: the sim's body b, b < 8) and, above them, the simulation's index (the same from
+// every marker); the first mark appends the block to the list of parity par.
+__device__ __forceinline__ void mark_block(const KArgs &A, long long gblk, unsigned bits) {
+    if (atomicOr(&A.nflag[gblk], bits) == 0u) {
         const int pos = atomicAdd(&A.ncount[A.par], 1);
         A.nlist[(long long)A.par * A.nblk_total + pos] = (int)gblk;
     }
 }
 
-// mark the node blocks covering nodes [lo, hi] (inclusive) of a sim for its body `lb`
-// (index within the sim); called by one thread
+// mark the node blocks covering nodes [lo, hi] (inclusive) of sim `s` for its body B;
+// called by one thread
 template <int DIM>
-__device__ void mark_range(const KArgs &A, const SimDev &s, int lbody, const int *lo, const int *hi) {
+__device__ void mark_range(const KArgs &A, const SimDev &s, const BodyDev &B, const int *lo, const int *hi) {
+    const unsigned bits = (1u << B.local) | ((unsigned)B.sim << 8);
     constexpr int L = Cfg<DIM>::LNB;
     int b0[3] = {0, 0, 0}, b1[3] = {0, 0, 0};
 #pragma unroll
@@ -50,7 +52,7 @@ __device__ void mark_range(const KArgs &A, const SimDev &s, int lbody, const int
         for (int j = b0[1]; j <= b1[1]; j++)
             for (int i = b0[0]; i <= b1[0]; i++) {
                 long long lb = i + (long long)s.NBn[0] * (j + (long long)s.NBn[1] * k);
-                mark_block(A, s.nblk_off + lb, 1u << lbody);
+                mark_block(A, s.nblk_off + lb, bits);
             }
 }
 
@@ -776,7 +778,7 @@ void k_particles(KArgs A, int want_L) {
                         lo[d] = w.base[d];
                         hi[d] = w.base[d] + 2;
                     }
-                    mark_range<DIM>(A, s, B.local, lo, hi);
+                    mark_range<DIM>(A, s, B, lo, hi);
                     for_each_sep<DIM, R, false>(w, q.ext, [&](const int *off, R S, const R *G) {
                         int nijk[DIM];
 #pragma unroll
@@ -799,7 +801,7 @@ void k_particles(KArgs A, int want_L) {
                                 atomicAdd(&lacc[f * PSTR + i], (double)vals[f]);
                             }
                         } else {
-                            mark_range<DIM>(A, s, B.local, nijk, nijk);
+                            mark_range<DIM>(A, s, B, nijk, nijk);
                             scatter_global(nijk, S, G);
                         }
                     });
@@ -848,7 +850,7 @@ void k_particles(KArgs A, int want_L) {
                 int hi[DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; d++) hi[d] = lo[d] + ex[d] - 1;
-                mark_range<DIM>(A, s, B.local, lo, hi);
+                mark_range<DIM>(A, s, B, lo, hi);
             }
         }
         __syncthreads();
@@ -915,7 +917,7 @@ __global__ __launch_bounds__(256) void k_p2g_side(KArgs A) {
                 lo[d] = w.base[d];
                 hi[d] = w.base[d] + 2;
             }
-            mark_range<DIM>(A, s, B.local, lo, hi);
+            mark_range<DIM>(A, s, B, lo, hi);
             for_each_sep<DIM, R>(w, q.ext, [&](const int *off, R S, const R *G) {
                 int nijk[DIM];
                 for (int d = 0; d < DIM; d++) nijk[d] = w.base[d] + off[d];
@@ -923,7 +925,7 @@ __global__ __launch_bounds__(256) void k_p2g_side(KArgs A) {
             });
         } else if (mode == MODE_TUP) {
             PT::template tuples<DIM>(s, q, cpdi, [&](const int *nijk, R S, const R *G) {
-                mark_range<DIM>(A, s, B.local, nijk, nijk);
+                mark_range<DIM>(A, s, B, nijk, nijk);
                 scatter(nijk, S, G);
             });
         }
@@ -1005,17 +1007,9 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
     unsigned long long active_nodes = 0;
     for (int bi = blockIdx.x * wpb + wv; bi < cnt; bi += gridDim.x * wpb) {
         const long long gblk = A.nlist[(long long)par * A.nblk_total + bi];
-        int si = 0;
-        if (A.nsim > 1) {  // binary search: last sim with nblk_off <= gblk
-            int lo = 0, hi = A.nsim - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (A.sims[mid].nblk_off <= gblk) lo = mid;
-                else hi = mid - 1;
-            }
-            si = lo;
-        }
-        si = uniform(si);
+        // the block's flag: the bodies that scattered into it and the simulation's index
+        const unsigned flag = (NBM > 1 || A.nsim > 1) ? (unsigned)uniform((int)A.nflag[gblk]) : 1u;
+        const int si = (int)(flag >> 8);
         const SimDev &s = A.sims[si];
         const long long lb = gblk - s.nblk_off;
         int ijk[3] = {0, 0, 0};
@@ -1043,7 +1037,7 @@ __global__ __launch_bounds__(256) void k_grid(KArgs A) {
         const long long addr = lb * 64 + lane;
         // bodies of the sim that scattered into this block: the others hold zero accumulators
         // here, which are not read (or re-zeroed); their outputs are still written (zeros)
-        const unsigned touched = NBM == 1 ? 1u : (unsigned)uniform((int)A.nflag[gblk]);
+        const unsigned touched = NBM == 1 ? 1u : flag & 0xffu;
         if (valid) {
             const int nb = s.nbody;
             // f0: the force before contacts act, which the frictional box takes its normal
