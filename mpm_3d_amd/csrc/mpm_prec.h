@@ -277,10 +277,6 @@ struct Prec32 {
     __device__ __forceinline__ static int window(const SimDev &s, const State<DIM> &q, Win<DIM, float> &w,
                                                  bool &cpdi) {
         cpdi = false;
-        bool in = true;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) in = in && inside(q.c[d], q.r[d], s.N[d]);
-        if (!in) return MODE_LEFT;
         // CPDI off (cpdi_spec 0, or a corner outside the domain): both "corners" at the point
         // and the gradient slots set to -/+ extent / h, as in point_window; on a node line this
         // is the one-sided derivative of the per-tuple path below
@@ -294,8 +290,14 @@ struct Prec32 {
             cp[d] = q.r[d] + a;
             cin = cin && inside(q.c[d], cm[d], s.N[d]) && inside(q.c[d], cp[d], s.N[d]);
         }
-        if (!cin) {  // a corner leaves the domain: CPDI off
-            off = true;
+        // both corners inside put the point, which lies between them, inside too (the domain
+        // is a box; inside() is monotone in r); otherwise the point is tested on its own
+        if (!cin) {
+            bool in = true;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) in = in && inside(q.c[d], q.r[d], s.N[d]);
+            if (!in) return MODE_LEFT;
+            off = true;  // a corner leaves the domain: CPDI off
 #pragma unroll
             for (int d = 0; d < DIM; d++) cm[d] = cp[d] = q.r[d];
         }
