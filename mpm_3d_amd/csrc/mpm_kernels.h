@@ -284,15 +284,16 @@ __device__ __forceinline__ void side_done(unsigned *rare_n, int k) {
 // second load trip (the dual cells first, then each lane's point). Measured on the 16M column:
 // FP64 P2G 1.33 ms with order 2 against 1.39 in slot order; FP32 P2G 1.26 ms with order 2
 // against 1.22 in slot order and 1.20 in slot order with the next point prefetched.
-// G2P pass in 3D: load the next particle block's node patch by LDS DMA during this block
-#ifndef MPM_G2P_ASYNC_PATCH
-#define MPM_G2P_ASYNC_PATCH 1
-#endif
 #ifndef MPM_P2G_ORDER32
 #define MPM_P2G_ORDER32 0
 #endif
 #ifndef MPM_P2G_ORDER64
 #define MPM_P2G_ORDER64 2
+#endif
+// G2P pass: load the next particle block's node patch by LDS DMA while this block's points
+// run (measured on the 16M column: G2P FP32 0.865 -> 0.838 ms, FP64 1.670 -> 1.532 ms)
+#ifndef MPM_G2P_ASYNC_PATCH
+#define MPM_G2P_ASYNC_PATCH 1
 #endif
 #ifndef MPM_DYN_BLOCKS
 #define MPM_DYN_BLOCKS 1
