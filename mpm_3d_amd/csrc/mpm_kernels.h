@@ -518,9 +518,10 @@ void k_particles(KArgs A, int want_L) {
         const int lane = tid & 63;
         if (ASYNC) {
             // the workgroup's next block: DMA its patch into graw (after the prefetch above,
-            // whose wait must not include these loads). Each instruction moves the two
-            // x-adjacent node blocks (contiguous in storage) of one field, 16 bytes a lane;
-            // node blocks outside the simulation's block grid are not read.
+            // whose wait must not include these loads). A group moves the two x-adjacent
+            // node blocks (contiguous in storage) of one field, 16 bytes a lane: one group
+            // per instruction in FP64, two in FP32; node blocks outside the simulation's
+            // block grid are not read.
             // (the block list and descriptors are read through the constant address space:
             // scalar loads, whose wait does not take in the vector loads in flight)
             pre = false;
@@ -546,17 +547,22 @@ void k_particles(KArgs A, int want_L) {
 #pragma unroll
                 for (int d = 0; d < DIM; d++) bc[d] = (org2[d] + NSHIFT) / Cfg<DIM>::NB;
                 constexpr int PER = 16 / (int)sizeof(GO);  // values per lane
-                const int half = 64 / PER;                 // lanes per node block
-                const int xb = bc[0] + (lane >= half ? 1 : 0);
+                constexpr int LPJ = 128 / PER;             // lanes per (pair, field): two node blocks
+                constexpr int JPI = 64 / LPJ;              // (pair, field) groups per instruction
+                const int jl = lane / LPJ, l2 = lane - jl * LPJ;
+                const int xb = bc[0] + (l2 >= LPJ / 2 ? 1 : 0);
                 const GO *src0 = sizeof(GO) == 8 ? reinterpret_cast<const GO *>(A.out) : reinterpret_cast<const GO *>(A.out32);
-                for (int j = tid >> 6; j < NPAIR * NOUT; j += (int)(blockDim.x >> 6)) {
+                for (int j0 = (tid >> 6) * JPI; j0 < NPAIR * NOUT; j0 += (int)(blockDim.x >> 6) * JPI) {
+                    // graw[j * 128] holds group j, so the groups j0 .. j0 + JPI - 1 of one
+                    // instruction land contiguously, lane after lane
+                    const int j = j0 + jl;
                     const int pair = j / NOUT, f = j - pair * NOUT;
                     const int by = bc[1] + (pair & 1), bz = DIM == 3 ? bc[DIM - 1] + (pair >> 1) : 0;
-                    if (lane < 2 * half && xb < s2.NBn[0] && by < s2.NBn[1] && (DIM == 2 || bz < s2.NBn[DIM - 1])) {
+                    if (j < NPAIR * NOUT && xb < s2.NBn[0] && by < s2.NBn[1] && (DIM == 2 || bz < s2.NBn[DIM - 1])) {
                         const long long blk = bc[0] + (long long)s2.NBn[0] * (by + (long long)s2.NBn[1] * bz);
-                        const GO *g = src0 + (long long)f * A.nt + B2.grid_base + blk * 64 + (long long)lane * PER;
+                        const GO *g = src0 + (long long)f * A.nt + B2.grid_base + blk * 64 + (long long)l2 * PER;
                         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)g,
-                                                         (__attribute__((address_space(3))) void *)(graw + j * 128),
+                                                         (__attribute__((address_space(3))) void *)(graw + j0 * 128),
                                                          16, 0, 0);
                     }
                 }
