@@ -145,6 +145,16 @@ __host__ __device__ __forceinline__ void for_each_sep(const Win<DIM, R> &w, R ex
     }
 }
 
+// 1 / (2^DIM ext / 2), the gradient scale of the separable window: correctly rounded in
+// FP64; in FP32 the hardware reciprocal (1 ulp), one instruction where the rounded division
+// takes ten
+template <int DIM, class R>
+__device__ __forceinline__ R grad_scale(R ext) {
+    const R den = (R)(1 << DIM) * (R)0.5 * ext;
+    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rcpf(den);
+    else return (R)1 / den;
+}
+
 // Row coefficients of the separable window: along a row (fixed c1, c2) the x-slot a has
 //   S = sw0[a] * Su,  G0 = dw0[a] * gx,  G1 = sw0[a] * gy,  G2 = sw0[a] * gz
 template <class R> struct RowW {
@@ -636,7 +646,7 @@ void k_particles(KArgs A, int want_L) {
                 }
                 if (lds_path) {
                     // row-factored gather: per row, U = sum sw0 v, DV = sum sw0 dv, LX = sum dw0 v
-                    const R gscale = (R)1 / ((R)(1 << DIM) * (R)0.5 * q.ext);
+                    const R gscale = grad_scale<DIM, R>(q.ext);
                     const int i0 = patch_index<DIM>(w.base, org);
 #pragma unroll
                     for (int c2 = 0; c2 < NZ; c2++) {
@@ -745,7 +755,7 @@ void k_particles(KArgs A, int want_L) {
                     // form too: scalar FP32 and FP64 FMAs issue at one rate, and the LDS
                     // accumulators are FP64, so this saves a conversion per value)
                     using D = double;
-                    const D gscale = (D)((R)1 / ((R)(1 << DIM) * (R)0.5 * q.ext));
+                    const D gscale = (D)grad_scale<DIM, R>(q.ext);
                     const D dm = pv.m;
                     D dmv[DIM], dVT[DIM][DIM], sx0[3], dx0[3];
 #pragma unroll

@@ -451,7 +451,9 @@ struct Prec32 {
         float tr = Lp[0][0] + Lp[1][1];
         if (DIM == 3) tr = tr + Lp[DIM - 1][DIM - 1];
         q.lnJ = q.lnJ + dtf * tr;
-        q.V = q.V * expf(dtf * tr);  // the fused pass scatters with the updated volume
+        // the fused pass scatters with the updated volume and the sand update reads it; the
+        // state holds V = 0 where neither needs it (the split G2P of an elastic body)
+        if (q.V != 0.0f) q.V = q.V * expf(dtf * tr);
         if (B.has_material) {
             float L3[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
