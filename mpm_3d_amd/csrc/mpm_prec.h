@@ -94,7 +94,17 @@ struct Prec64 {
         }
         double tr = Lp[0][0] + Lp[1][1];
         if (DIM == 3) tr = tr + Lp[DIM - 1][DIM - 1];
-        q.V = q.V * exp(dt * tr);  // explicit_usl.cpp:338
+        // V *= exp(dt tr L) (explicit_usl.cpp:338). For |dt tr L| <= 2^-7 exp is its Taylor
+        // polynomial through degree 6 (remainder below 2^-60 relative), by Horner; exp()
+        // otherwise. Both are within two ulps of the exact value.
+        const double xe = dt * tr;
+        double ex;
+        if (fabs(xe) <= 0x1p-7)
+            ex = fma(xe, fma(xe, fma(xe, fma(xe, fma(xe, fma(xe, 1.0 / 720, 1.0 / 120), 1.0 / 24), 1.0 / 6), 0.5), 1.0),
+                     1.0);
+        else
+            ex = exp(xe);
+        q.V = q.V * ex;
         if (B.has_material) {
             double L3[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
