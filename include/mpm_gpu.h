@@ -226,7 +226,15 @@ int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
  *                          gradient for mpm_gpu_download_aux (9 doubles per point slot,
  *                          allocated at the first step and not counted by mpm_gpu_footprint);
  *                          0: not kept, the plane is freed and mpm_gpu_download_aux's L
- *                          returns MPM_ERR_STATE */
+ *                          returns MPM_ERR_STATE
+ *   MPM_OPT_CELL_SORT      1 (default where it fits): the sort also orders each particle
+ *                          block's points by dual cell within their octant, so that the
+ *                          particle passes' LDS accesses fall on distinct banks; it keeps 512
+ *                          bytes of counters per (block, octant) of every body's
+ *                          particle-block grid plus a byte per point slot (counted by
+ *                          mpm_gpu_footprint), and is off (setting 1 returns MPM_ERR_ARG) on
+ *                          handles where the counters would exceed 4 GiB; 0: blocks and
+ *                          octants only */
 #define MPM_OPT_GRAPH 3
 #define MPM_OPT_AUTOTUNE 4
 #define MPM_OPT_PASS_GRID 5
@@ -235,6 +243,7 @@ int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
 #define MPM_OPT_P2G_GRID 8
 #define MPM_OPT_NODE_RECORD 9
 #define MPM_OPT_KEEP_L 10
+#define MPM_OPT_CELL_SORT 11
 int mpm_gpu_set_option(mpm_gpu_handle *h, int key, int value);
 int mpm_gpu_get_option(mpm_gpu_handle *h, int key, int *value);
 

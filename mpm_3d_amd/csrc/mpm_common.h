@@ -207,6 +207,19 @@ __device__ __forceinline__ int body_of_index(const KArgs &A, long long o) {
     return lo;
 }
 
+// Lane of the P2G lane order for a point of in-block dual cell `dual` (KB = 2 or 3 bits per
+// dimension, 64 keys). Order 2 is also the order of the dual cells inside a (block, octant)
+// bucket of the cell-ordered sort.
+template <int DIM, int ORDER>
+__device__ __forceinline__ int lane_key(const int *dual) {
+    constexpr int KB = Cfg<DIM>::PB == 4 ? 2 : 3;
+    static_assert(KB * DIM == 6, "one key per lane of a wavefront");
+    constexpr int m = (1 << KB) - 1;
+    const int a0 = dual[0] & m, a1 = dual[1] & m, a2 = DIM == 3 ? dual[DIM - 1] & m : 0;
+    if (DIM == 3) return ORDER == 1 ? a0 | a1 << KB | a2 << (2 * KB) : a0 | a2 << KB | a1 << (2 * KB);
+    return ORDER == 1 ? a0 | a1 << KB : a1 | a0 << KB;
+}
+
 // sim-local node (i,j,k), -1 <= i <= N+1, -> offset inside the sim's node-block storage.
 // Storage is shifted by two nodes: the zero-weight slots of a 3-node window that hang
 // over the domain edge (nodes -1 and N+1) have a home (those padding nodes stay zero),

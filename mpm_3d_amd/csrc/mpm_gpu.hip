@@ -67,7 +67,13 @@ struct Layout {  // sizes derived from the descriptors, shared by create and foo
     long long np = 0, nt = 0, nblk_total = 0, nkeys = 0, npblk_total = 0;
     long long crec_total = 0;  // contact record elements (ContactDev::rec), allocated on demand
     bool any_b1 = false;
+    bool cell_fit = false;  // the cell-ordered sort's counters fit their budget (CELL_SORT_BUDGET)
 };
+
+// The cell-ordered sort keeps 64 counters and 64 offsets (4 bytes each) per (block, octant)
+// key over the whole particle-block grid of every body; handles whose counters would exceed
+// this many bytes sort by (block, octant) only.
+constexpr long long CELL_SORT_BUDGET = 4ll << 30;
 
 // IsotropicLinearElasticity::init (isolin.cpp:34-38); Sand_SachithLocal::init computes the
 // same moduli and keeps its six properties (sand_sachithlocal.cpp:58-117)
@@ -196,6 +202,7 @@ int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
     Lo.nkeys = pblk_off * (1ll << Lo.dim) + 1;  // (block, octant) keys + inactive bucket
     if (Lo.np >= (1ll << 31) - 1) return fail(MPM_ERR_ARG, "at most 2^31-2 points per handle");
     if (Lo.nkeys >= (1ll << 31) - 1) return fail(MPM_ERR_ARG, "grid too large for one handle");
+    Lo.cell_fit = (Lo.nkeys - 1) * 64 * 8 <= CELL_SORT_BUDGET;
     return MPM_OK;
 }
 
@@ -211,7 +218,9 @@ long long footprint_bytes(const Layout &Lo, int precision) {
     b += Lo.nt * (nacc * 8 + nout * esz);  // grid
     b += Lo.nblk_total * (4 + 2 * 4);      // node-block flags + two lists
     b += Lo.nkeys * (4 + 4);               // histogram, offsets
+    if (Lo.cell_fit) b += (Lo.nkeys - 1) * 64 * (4 + 4) + Lo.np;  // cell counters, offsets, cells
     b += Lo.npblk_total * sizeof(PBlk);    // block list
+    b += (Lo.npblk_total + 1) * (4 + 4);   // block flags, ranks
     b += Lo.np * 4 + 16;                   // side list of the particle passes
     b += 8;                                // block counter and non-finite flags
     return b;
@@ -243,9 +252,14 @@ struct mpm_gpu_handle {
     int *nlist = nullptr;
     int *ncount = nullptr;
     unsigned *bcount = nullptr, *boff = nullptr;
+    unsigned *fcnt = nullptr, *foff = nullptr;  // cell-ordered sort: per (block, octant) key, 64 each
+    uint8_t *pcell = nullptr;                   // cell-ordered sort: a slot's dual cell in its block
+    bool cell_sort = false;                     // MPM_OPT_CELL_SORT
     std::vector<unsigned *> scan_tmp;  // per level: block sums, their scan
     unsigned *inact_count = nullptr, *inact_off = nullptr;  // inactive slots per sort workgroup
     std::vector<unsigned *> scan_tmp_i;
+    unsigned *bflag = nullptr, *bpos = nullptr;  // non-empty particle blocks, their ranks
+    std::vector<unsigned *> scan_tmp_b;
     std::vector<long long> scan_n;
     PBlk *plist = nullptr;
     int *npblk = nullptr;
@@ -403,21 +417,29 @@ int launch_sort(mpm_gpu_handle *h) {
     int rc = timer_begin(h, KC_SORT, t);
     if (rc) return rc;
     const int gp = (int)std::max(1ll, ceil_div(np, 256));
+    const bool cells = h->cell_sort;
     hipLaunchKernelGGL((k_bin_count<DIM, PT>), dim3(gp), dim3(256), 0, h->stream, A, h->bcount, h->pkey, h->prank,
-                       nk - 1, h->inact_count);
+                       nk - 1, h->inact_count, cells ? h->fcnt : nullptr, h->pcell);
     scan_exclusive(h, h->bcount, h->boff, nk, h->scan_tmp);             // bucket offsets
     scan_exclusive(h, h->inact_count, h->inact_off, gp, h->scan_tmp_i);  // inactive ranks per workgroup
-    hipLaunchKernelGGL(k_make_plist, dim3((unsigned)ceil_div(nk, 256)), dim3(256), 0, h->stream, A, h->boff,
-                       h->plist, h->npblk, nk, 1 << DIM);
+    const long long nblk = h->lo.npblk_total;
+    hipLaunchKernelGGL(k_block_flags, dim3((unsigned)ceil_div(nblk + 1, 256)), dim3(256), 0, h->stream, h->boff,
+                       h->bflag, nblk, 1 << DIM);
+    scan_exclusive(h, h->bflag, h->bpos, nblk + 1, h->scan_tmp_b);
+    hipLaunchKernelGGL(k_make_plist, dim3((unsigned)ceil_div(nblk + 1, 256)), dim3(256), 0, h->stream, A, h->boff,
+                       h->bpos, h->plist, h->npblk, nblk, 1 << DIM);
+    if (cells)
+        hipLaunchKernelGGL(k_cell_offsets<DIM>, dim3((unsigned)std::max(1ll, h->lo.npblk_total)), dim3(64 << DIM), 0,
+                           h->stream, h->plist, h->npblk, h->boff, h->fcnt, h->foff);
     hipLaunchKernelGGL(k_clear, dim3((unsigned)ceil_div(nk, 256)), dim3(256), 0, h->stream, h->bcount, nk);
     if constexpr (PT::FP32) {
         hipLaunchKernelGGL((k_bin_scatter<uint32_t, PW<DIM>::N>), dim3(gp), dim3(256), 0, h->stream, np, h->Pw, h->Pw2,
                            h->act, h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff, (int)(nk - 1),
-                           h->inact_off);
+                           h->inact_off, cells ? h->foff : nullptr, h->pcell);
     } else {
         hipLaunchKernelGGL((k_bin_scatter<double, PF<DIM>::N>), dim3(gp), dim3(256), 0, h->stream, np, h->P, h->P2,
                            h->act, h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff, (int)(nk - 1),
-                           h->inact_off);
+                           h->inact_off, cells ? h->foff : nullptr, h->pcell);
     }
     swap_buffers(h);
     HIP_TRY(hipGetLastError());
@@ -784,6 +806,12 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     TRY(dalloc(&h->ncount, 2));
     TRY(dalloc(&h->bcount, L.nkeys));
     TRY(dalloc(&h->boff, L.nkeys));
+    if (L.cell_fit) {
+        TRY(dalloc(&h->fcnt, (L.nkeys - 1) * 64));
+        TRY(dalloc(&h->foff, (L.nkeys - 1) * 64));
+        TRY(dalloc(&h->pcell, L.np));
+        h->cell_sort = true;
+    }
     TRY(dalloc(&h->plist, L.npblk_total));
     TRY(dalloc(&h->npblk, 1));
     TRY(dalloc(&h->stats, 4));
@@ -809,6 +837,9 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     TRY(dalloc(&h->inact_count, gp_sort));
     TRY(dalloc(&h->inact_off, gp_sort));
     TRY(alloc_scan_tmp(gp_sort, h->scan_tmp_i));
+    TRY(dalloc(&h->bflag, L.npblk_total + 1));
+    TRY(dalloc(&h->bpos, L.npblk_total + 1));
+    TRY(alloc_scan_tmp(L.npblk_total + 1, h->scan_tmp_b));
     // zero state: accumulators must start at zero, flags clear, counters zero
     e = hipMemsetAsync(h->acc, 0, sizeof(double) * L.nt * nacc, h->stream);
     if (e == hipSuccess)
@@ -817,6 +848,8 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     if (e == hipSuccess) e = hipMemsetAsync(h->nflag, 0, sizeof(unsigned) * std::max(1ll, L.nblk_total), h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->ncount, 0, sizeof(int) * 2, h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->bcount, 0, sizeof(unsigned) * L.nkeys, h->stream);
+    if (e == hipSuccess && h->fcnt)
+        e = hipMemsetAsync(h->fcnt, 0, sizeof(unsigned) * std::max(1ll, (L.nkeys - 1) * 64), h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->npblk, 0, sizeof(int), h->stream);
     if (e == hipSuccess)
         e = f32 ? hipMemsetAsync(h->Pw, 0, sizeof(uint32_t) * std::max(1ll, L.np * nf), h->stream)
@@ -1116,6 +1149,20 @@ int mpm_gpu_set_option(mpm_gpu_handle *h, int key, int value) {
             h->autotune = value != 0;
             h->tuned = false;
             return MPM_OK;
+        case MPM_OPT_CELL_SORT:
+            if (value != 0 && !h->fcnt)
+                return fail(MPM_ERR_ARG, "the cell-ordered sort's counters exceed their budget on this handle");
+            if ((value != 0) != h->cell_sort) {
+                // captured sort periods bake the sort's launches
+                HIP_TRY(hipStreamSynchronize(h->stream));
+                for (hipGraphExec_t &g : h->graph)
+                    if (g) {
+                        HIP_TRY(hipGraphExecDestroy(g));
+                        g = nullptr;
+                    }
+            }
+            h->cell_sort = value != 0;
+            return MPM_OK;
         case MPM_OPT_NODE_RECORD:
             if (value != 0 && !h->rec) {
                 const long long nr = (1LL + 2 * h->lo.dim) * std::max(1ll, h->lo.nt);
@@ -1157,6 +1204,7 @@ int mpm_gpu_get_option(mpm_gpu_handle *h, int key, int *value) {
         case MPM_OPT_FUSED_PASS: *value = h->fused; return MPM_OK;
         case MPM_OPT_GRAPH: *value = h->use_graph; return MPM_OK;
         case MPM_OPT_AUTOTUNE: *value = h->autotune ? (h->tuned ? 2 : 1) : 0; return MPM_OK;
+        case MPM_OPT_CELL_SORT: *value = h->cell_sort; return MPM_OK;
         case MPM_OPT_PASS_GRID: *value = h->pass_grid; return MPM_OK;
         case MPM_OPT_P2G_GRID: *value = h->p2g_grid; return MPM_OK;
         case MPM_OPT_NODE_RECORD: *value = h->record; return MPM_OK;
@@ -1671,12 +1719,14 @@ int mpm_gpu_destroy(mpm_gpu_handle *h) {
     void *ptrs[] = {h->d_sims, h->d_bodies, h->d_contacts, h->P, h->P2, h->Pw, h->Pw2, h->act, h->act2, h->pid,
                     h->pid2, h->pkey, h->prank, h->acc, h->out, h->out32, h->nflag, h->nlist, h->ncount, h->bcount, h->boff,
                     h->plist, h->npblk, h->Lout, h->stats, h->stage, h->rec, h->crec, h->rare, h->rare_n,
-                    h->work, h->nonfin};
+                    h->work, h->nonfin, h->fcnt, h->foff, h->pcell, h->bflag, h->bpos};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (unsigned *p : h->scan_tmp)
         if (p) (void)hipFree(p);
     for (unsigned *p : h->scan_tmp_i)
+        if (p) (void)hipFree(p);
+    for (unsigned *p : h->scan_tmp_b)
         if (p) (void)hipFree(p);
     if (h->inact_count) (void)hipFree(h->inact_count);
     if (h->inact_off) (void)hipFree(h->inact_off);

@@ -317,17 +317,6 @@ template <int DIM, bool DO_G2P, bool DO_P2G, bool FP32>
 constexpr bool dyn_blocks() {
     return DIM == 3 && DO_G2P != DO_P2G && (DO_P2G || !FP32);
 }
-// Lane of the P2G lane order for a point of in-block dual cell `dual` (KB = 2 or 3 bits per
-// dimension, 64 keys).
-template <int DIM, int ORDER>
-__device__ __forceinline__ int lane_key(const int *dual) {
-    constexpr int KB = Cfg<DIM>::PB == 4 ? 2 : 3;
-    static_assert(KB * DIM == 6, "one key per lane of a wavefront");
-    constexpr int m = (1 << KB) - 1;
-    const int a0 = dual[0] & m, a1 = dual[1] & m, a2 = DIM == 3 ? dual[DIM - 1] & m : 0;
-    if (DIM == 3) return ORDER == 1 ? a0 | a1 << KB | a2 << (2 * KB) : a0 | a2 << KB | a1 << (2 * KB);
-    return ORDER == 1 ? a0 | a1 << KB : a1 | a0 << KB;
-}
 
 // Called by the whole wavefront: lane L holds a point with lane-order key `key` (-1: no
 // point; -2: a point without a key, e.g. one leaving the domain). Returns, for every lane,

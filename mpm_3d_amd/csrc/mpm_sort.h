@@ -6,15 +6,22 @@
 // patch. Inside a block the key orders points by the octant of their position within the
 // dual cell first: the 64 consecutive points a wavefront takes then sit in ~64 different
 // dual cells, so their LDS atomics hit different nodes (no same-address serialisation)
-// while the loads stay coalesced.
+// while the loads stay coalesced. With the cell-ordered sort the points of each (block,
+// octant) bucket are further ordered by dual cell, in the P2G lane order (x + 4z + 16y in
+// 3D): a wavefront over a bucket with one point per dual cell then places every point on
+// the lane of its cell, whose LDS accesses fall on distinct banks, without the pass
+// reordering its lanes.
 // The reference has no counterpart: it scatters straight from point order through COO
 // tuples (src/algebra/mpm_sparse.cpp:30-44).
 //
-//   k_bin_count   key per slot (inactive points go to the last bucket), wavefront-
-//                 aggregated atomic histogram, rank inside the bucket
-//   scan          exclusive prefix sum of the histogram (k_scan_*)
-//   k_make_plist  one PBlk per non-empty bucket; k_clear then clears the histogram
-//   k_bin_scatter moves every particle plane to its sorted slot
+//   k_bin_count    key per slot (inactive points go to the last bucket), wavefront-
+//                  aggregated atomic histogram, rank inside the bucket (cell-ordered: rank
+//                  inside the bucket's dual cell, from a per-cell counter)
+//   scan           exclusive prefix sum of the histogram (k_scan_*)
+//   k_make_plist   one PBlk per non-empty block, in block order (k_block_flags, a scan);
+//                  k_clear then clears the histogram
+//   k_cell_offsets (cell-ordered) per non-empty bucket, the prefix of its 64 cell counters
+//   k_bin_scatter  moves every particle plane to its sorted slot
 #pragma once
 
 #include "mpm_common.h"
@@ -38,13 +45,16 @@ __device__ __forceinline__ int body_of_block(const KArgs &A, long long key) {
 // atomics: the rank inside the workgroup here, the workgroup's count in inact_count
 // (scanned into per-workgroup offsets before the scatter).
 template <int DIM, class PT>
+// fcnt (cell-ordered sort, else null): 64 counters per (block, octant) key, all zero on entry
 __global__ __launch_bounds__(256) void k_bin_count(KArgs A, unsigned *count, int *pkey, int *prank,
-                                                   long long inactive_key, unsigned *inact_count) {
+                                                   long long inactive_key, unsigned *inact_count, unsigned *fcnt,
+                                                   uint8_t *pcell) {
     constexpr int PB = Cfg<DIM>::PB;
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) *const_cast<int *>(A.npblk) = 0;
     const bool live = p < A.np;
     long long key = inactive_key;
+    int cell = 0;
     if (live && A.act[p] && A.pid[p] >= 0) {
         const int b = body_of_index(A, A.pid[p]);
         const BodyDev &B = A.bodies[b];
@@ -58,6 +68,7 @@ __global__ __launch_bounds__(256) void k_bin_count(KArgs A, unsigned *count, int
                 stride *= s.PNB[d];
             }
             key = (B.pblk_off + lk) * (1 << DIM) + oct;
+            cell = lane_key<DIM, 2>(dual);
         } else {
             A.act[p] = 0;  // leaves the domain: deactivated (default_points.cpp:217-219)
         }
@@ -87,6 +98,11 @@ __global__ __launch_bounds__(256) void k_bin_count(KArgs A, unsigned *count, int
         base = __shfl(base, leader);
         if ((same >> lane) & 1ull) rank = (int)base + __popcll(same & ((1ull << lane) - 1ull));
         pending &= ~same;
+    }
+    // cell-ordered: the rank inside the point's dual cell of its bucket instead
+    if (fcnt && live && !inact) {
+        rank = (int)atomicAdd(&fcnt[key * 64 + cell], 1u);
+        pcell[p] = (uint8_t)cell;
     }
     if (live) {
         pkey[p] = (int)key;
@@ -129,24 +145,53 @@ __global__ __launch_bounds__(1024) void k_scan_add(unsigned *out, const unsigned
         if (i + k < n) out[i + k] += add;
 }
 
-// one entry per non-empty particle block (keys are block * 2^DIM + octant), in no
-// particular order (the passes take blocks by index); k_clear then clears the histogram
-__global__ __launch_bounds__(256) void k_make_plist(KArgs A, const unsigned *off, PBlk *plist, int *npblk,
-                                                    long long nkeys, int noct) {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nkeys) return;
-    if (k < nkeys - 1 && k % noct == 0) {
-        const unsigned b0 = off[k], b1 = off[k + noct];
-        if (b1 > b0) {
-            const int pos = atomicAdd(npblk, 1);
-            PBlk e;
-            e.key = (int)(k / noct);
-            e.start = (int)b0;
-            e.count = (int)(b1 - b0);
-            e.body = body_of_block(A, k / noct);
-            plist[pos] = e;
-        }
+// Block list, in block order: flag[b] = block b holds points (its 2^DIM buckets, keys
+// b * 2^DIM + octant, are not all empty); flag[nblk] = 0 so that the scan's last entry is
+// the number of listed blocks
+__global__ __launch_bounds__(256) void k_block_flags(const unsigned *off, unsigned *flag, long long nblk, int noct) {
+    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nblk) return;
+    flag[b] = b < nblk && off[(b + 1) * noct] > off[b * noct] ? 1u : 0u;
+}
+
+// one entry per non-empty particle block at its rank among them (pos: exclusive scan of
+// the flags), so the particle passes take blocks in key order; k_clear then clears the
+// histogram
+__global__ __launch_bounds__(256) void k_make_plist(KArgs A, const unsigned *off, const unsigned *pos, PBlk *plist,
+                                                    int *npblk, long long nblk, int noct) {
+    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) *npblk = (int)pos[nblk];
+    if (b >= nblk || pos[b + 1] == pos[b]) return;
+    const unsigned b0 = off[b * noct], b1 = off[(b + 1) * noct];
+    PBlk e;
+    e.key = (int)b;
+    e.start = (int)b0;
+    e.count = (int)(b1 - b0);
+    e.body = body_of_block(A, b);
+    plist[pos[b]] = e;
+}
+
+// Cell-ordered sort: for each (block, octant) bucket of the listed blocks (one workgroup per
+// block, one wavefront per octant, lane = dual cell in the lane order), the exclusive prefix
+// of the bucket's 64 cell counters, which places its points cell after cell; the counters
+// are cleared for the next sort. Workgroups past the listed blocks exit.
+template <int DIM>
+__global__ __launch_bounds__(64 << DIM) void k_cell_offsets(const PBlk *plist, const int *npblk, const unsigned *off,
+                                                            unsigned *fcnt, unsigned *foff) {
+    if ((int)blockIdx.x >= *npblk) return;
+    const long long k = (long long)plist[blockIdx.x].key * (1 << DIM) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (off[k + 1] == off[k]) return;  // empty bucket (wavefront-uniform): its counters are zero
+    const long long i = k * 64 + lane;
+    const unsigned c = fcnt[i];
+    unsigned incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
     }
+    foff[i] = incl - c;
+    fcnt[i] = 0u;
 }
 
 __global__ void k_clear_u64(unsigned long long *p, int n) {
@@ -162,12 +207,14 @@ template <class E, int NF>
 __global__ __launch_bounds__(256) void k_bin_scatter(long long np, const E *P, E *P2, const int8_t *act,
                                                      int8_t *act2, const int *pid, int *pid2, const int *pkey,
                                                      const int *prank, const unsigned *off, int inactive_key,
-                                                     const unsigned *inact_off) {
+                                                     const unsigned *inact_off, const unsigned *foff,
+                                                     const uint8_t *pcell) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     const int key = pkey[p];
     long long q = (long long)off[key] + prank[p];
     if (key == inactive_key) q += inact_off[blockIdx.x];  // same 256-slot blocks as k_bin_count
+    else if (foff) q += foff[(long long)key * 64 + pcell[p]];  // cell-ordered: the cell's start
     const int o = pid[p];
     if (o < 0) {  // unused capacity: nothing to move
         act2[q] = 0;

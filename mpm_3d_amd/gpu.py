@@ -40,6 +40,7 @@ MPM_OPT_GRAPH_CAPTURES = 7
 MPM_OPT_P2G_GRID = 8
 MPM_OPT_NODE_RECORD = 9
 MPM_OPT_KEEP_L = 10
+MPM_OPT_CELL_SORT = 11
 
 
 class MpmGpuError(RuntimeError):
