@@ -229,7 +229,7 @@ int mpm_gpu_set_sort_interval(mpm_gpu_handle *h, int interval);
  *                          returns MPM_ERR_STATE
  *   MPM_OPT_CELL_SORT      1 (default where it fits): the sort also orders each particle
  *                          block's points by dual cell within their octant, so that the
- *                          particle passes' LDS accesses fall on distinct banks; it keeps 512
+ *                          particle passes' LDS accesses fall on distinct banks; it keeps 256
  *                          bytes of counters per (block, octant) of every body's
  *                          particle-block grid plus a byte per point slot (counted by
  *                          mpm_gpu_footprint), and is off (setting 1 returns MPM_ERR_ARG) on

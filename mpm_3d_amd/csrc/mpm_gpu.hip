@@ -70,9 +70,9 @@ struct Layout {  // sizes derived from the descriptors, shared by create and foo
     bool cell_fit = false;  // the cell-ordered sort's counters fit their budget (CELL_SORT_BUDGET)
 };
 
-// The cell-ordered sort keeps 64 counters and 64 offsets (4 bytes each) per (block, octant)
-// key over the whole particle-block grid of every body; handles whose counters would exceed
-// this many bytes sort by (block, octant) only.
+// The cell-ordered sort keeps 64 counters (4 bytes each, turned into offsets and back to
+// zero by every sort) per (block, octant) key over the whole particle-block grid of every
+// body; handles whose counters would exceed this many bytes sort by (block, octant) only.
 constexpr long long CELL_SORT_BUDGET = 4ll << 30;
 
 // IsotropicLinearElasticity::init (isolin.cpp:34-38); Sand_SachithLocal::init computes the
@@ -202,7 +202,7 @@ int build_layout(const mpm_sim_desc *sd, int nsim, Layout &Lo) {
     Lo.nkeys = pblk_off * (1ll << Lo.dim) + 1;  // (block, octant) keys + inactive bucket
     if (Lo.np >= (1ll << 31) - 1) return fail(MPM_ERR_ARG, "at most 2^31-2 points per handle");
     if (Lo.nkeys >= (1ll << 31) - 1) return fail(MPM_ERR_ARG, "grid too large for one handle");
-    Lo.cell_fit = (Lo.nkeys - 1) * 64 * 8 <= CELL_SORT_BUDGET;
+    Lo.cell_fit = (Lo.nkeys - 1) * 64 * 4 <= CELL_SORT_BUDGET;
     return MPM_OK;
 }
 
@@ -218,7 +218,7 @@ long long footprint_bytes(const Layout &Lo, int precision) {
     b += Lo.nt * (nacc * 8 + nout * esz);  // grid
     b += Lo.nblk_total * (4 + 2 * 4);      // node-block flags + two lists
     b += Lo.nkeys * (4 + 4);               // histogram, offsets
-    if (Lo.cell_fit) b += (Lo.nkeys - 1) * 64 * (4 + 4) + Lo.np;  // cell counters, offsets, cells
+    if (Lo.cell_fit) b += (Lo.nkeys - 1) * 64 * 4 + Lo.np;  // cell counters, a slot's cell
     b += Lo.npblk_total * sizeof(PBlk);    // block list
     b += (Lo.npblk_total + 1) * (4 + 4);   // block flags, ranks
     b += Lo.np * 4 + 16;                   // side list of the particle passes
@@ -252,9 +252,9 @@ struct mpm_gpu_handle {
     int *nlist = nullptr;
     int *ncount = nullptr;
     unsigned *bcount = nullptr, *boff = nullptr;
-    unsigned *fcnt = nullptr, *foff = nullptr;  // cell-ordered sort: per (block, octant) key, 64 each
-    uint8_t *pcell = nullptr;                   // cell-ordered sort: a slot's dual cell in its block
-    bool cell_sort = false;                     // MPM_OPT_CELL_SORT
+    unsigned *fcnt = nullptr;  // cell-ordered sort: per (block, octant) key, 64 counters / offsets
+    uint8_t *pcell = nullptr;  // cell-ordered sort: a slot's dual cell in its block
+    bool cell_sort = false;    // MPM_OPT_CELL_SORT
     std::vector<unsigned *> scan_tmp;  // per level: block sums, their scan
     unsigned *inact_count = nullptr, *inact_off = nullptr;  // inactive slots per sort workgroup
     std::vector<unsigned *> scan_tmp_i;
@@ -409,6 +409,10 @@ void scan_exclusive(mpm_gpu_handle *h, const unsigned *in, unsigned *outp, long 
     }
 }
 
+// workgroups of the cell-ordered sort's per-block kernels (grid-stride over the block list,
+// whose length is known only on the device)
+unsigned cell_wgs(const mpm_gpu_handle *h) { return (unsigned)std::min(4096ll, std::max(1ll, h->lo.npblk_total)); }
+
 template <int DIM, class PT>
 int launch_sort(mpm_gpu_handle *h) {
     const long long np = h->lo.np, nk = h->lo.nkeys;
@@ -429,18 +433,21 @@ int launch_sort(mpm_gpu_handle *h) {
     hipLaunchKernelGGL(k_make_plist, dim3((unsigned)ceil_div(nblk + 1, 256)), dim3(256), 0, h->stream, A, h->boff,
                        h->bpos, h->plist, h->npblk, nblk, 1 << DIM);
     if (cells)
-        hipLaunchKernelGGL(k_cell_offsets<DIM>, dim3((unsigned)std::max(1ll, h->lo.npblk_total)), dim3(64 << DIM), 0,
-                           h->stream, h->plist, h->npblk, h->boff, h->fcnt, h->foff);
+        hipLaunchKernelGGL(k_cell_offsets<DIM>, dim3(cell_wgs(h)), dim3(64 << DIM), 0, h->stream, h->plist, h->npblk,
+                           h->boff, h->fcnt);
     hipLaunchKernelGGL(k_clear, dim3((unsigned)ceil_div(nk, 256)), dim3(256), 0, h->stream, h->bcount, nk);
     if constexpr (PT::FP32) {
         hipLaunchKernelGGL((k_bin_scatter<uint32_t, PW<DIM>::N>), dim3(gp), dim3(256), 0, h->stream, np, h->Pw, h->Pw2,
                            h->act, h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff, (int)(nk - 1),
-                           h->inact_off, cells ? h->foff : nullptr, h->pcell);
+                           h->inact_off, cells ? h->fcnt : nullptr, h->pcell);
     } else {
         hipLaunchKernelGGL((k_bin_scatter<double, PF<DIM>::N>), dim3(gp), dim3(256), 0, h->stream, np, h->P, h->P2,
                            h->act, h->act2, h->pid, h->pid2, h->pkey, h->prank, h->boff, (int)(nk - 1),
-                           h->inact_off, cells ? h->foff : nullptr, h->pcell);
+                           h->inact_off, cells ? h->fcnt : nullptr, h->pcell);
     }
+    if (cells)
+        hipLaunchKernelGGL(k_cell_clear<DIM>, dim3(cell_wgs(h)), dim3(64 << DIM), 0, h->stream, h->plist, h->npblk,
+                           h->fcnt);
     swap_buffers(h);
     HIP_TRY(hipGetLastError());
     return timer_end(h, t);
@@ -808,7 +815,6 @@ int create_from_layout(Layout &&lo, int precision, mpm_gpu_handle **out) {
     TRY(dalloc(&h->boff, L.nkeys));
     if (L.cell_fit) {
         TRY(dalloc(&h->fcnt, (L.nkeys - 1) * 64));
-        TRY(dalloc(&h->foff, (L.nkeys - 1) * 64));
         TRY(dalloc(&h->pcell, L.np));
         h->cell_sort = true;
     }
@@ -1719,7 +1725,7 @@ int mpm_gpu_destroy(mpm_gpu_handle *h) {
     void *ptrs[] = {h->d_sims, h->d_bodies, h->d_contacts, h->P, h->P2, h->Pw, h->Pw2, h->act, h->act2, h->pid,
                     h->pid2, h->pkey, h->prank, h->acc, h->out, h->out32, h->nflag, h->nlist, h->ncount, h->bcount, h->boff,
                     h->plist, h->npblk, h->Lout, h->stats, h->stage, h->rec, h->crec, h->rare, h->rare_n,
-                    h->work, h->nonfin, h->fcnt, h->foff, h->pcell, h->bflag, h->bpos};
+                    h->work, h->nonfin, h->fcnt, h->pcell, h->bflag, h->bpos};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (unsigned *p : h->scan_tmp)

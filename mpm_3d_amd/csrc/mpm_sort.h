@@ -20,8 +20,10 @@
 //   scan           exclusive prefix sum of the histogram (k_scan_*)
 //   k_make_plist   one PBlk per non-empty block, in block order (k_block_flags, a scan);
 //                  k_clear then clears the histogram
-//   k_cell_offsets (cell-ordered) per non-empty bucket, the prefix of its 64 cell counters
+//   k_cell_offsets (cell-ordered) per non-empty bucket, the prefix of its 64 cell counters,
+//                  in place
 //   k_bin_scatter  moves every particle plane to its sorted slot
+//   k_cell_clear   (cell-ordered) zeroes the listed blocks' cell counters for the next sort
 #pragma once
 
 #include "mpm_common.h"
@@ -171,27 +173,36 @@ __global__ __launch_bounds__(256) void k_make_plist(KArgs A, const unsigned *off
     plist[pos[b]] = e;
 }
 
-// Cell-ordered sort: for each (block, octant) bucket of the listed blocks (one workgroup per
-// block, one wavefront per octant, lane = dual cell in the lane order), the exclusive prefix
-// of the bucket's 64 cell counters, which places its points cell after cell; the counters
-// are cleared for the next sort. Workgroups past the listed blocks exit.
+// Cell-ordered sort: for each (block, octant) bucket of the listed blocks (a workgroup per
+// block, grid-stride over the list; one wavefront per octant, lane = dual cell in the lane
+// order), the exclusive prefix of the bucket's 64 cell counters, written over them: it
+// places the bucket's points cell after cell.
 template <int DIM>
 __global__ __launch_bounds__(64 << DIM) void k_cell_offsets(const PBlk *plist, const int *npblk, const unsigned *off,
-                                                            unsigned *fcnt, unsigned *foff) {
-    if ((int)blockIdx.x >= *npblk) return;
-    const long long k = (long long)plist[blockIdx.x].key * (1 << DIM) + (threadIdx.x >> 6);
+                                                            unsigned *fcnt) {
     const int lane = threadIdx.x & 63;
-    if (off[k + 1] == off[k]) return;  // empty bucket (wavefront-uniform): its counters are zero
-    const long long i = k * 64 + lane;
-    const unsigned c = fcnt[i];
-    unsigned incl = c;
+    const int nb = *npblk;
+    for (int e = blockIdx.x; e < nb; e += gridDim.x) {
+        const long long k = (long long)plist[e].key * (1 << DIM) + (threadIdx.x >> 6);
+        if (off[k + 1] == off[k]) continue;  // empty bucket (wavefront-uniform): its counters are zero
+        const long long i = k * 64 + lane;
+        const unsigned c = fcnt[i];
+        unsigned incl = c;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const unsigned t = __shfl_up(incl, d);
-        if (lane >= d) incl += t;
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned t = __shfl_up(incl, d);
+            if (lane >= d) incl += t;
+        }
+        fcnt[i] = incl - c;
     }
-    foff[i] = incl - c;
-    fcnt[i] = 0u;
+}
+
+// Cell-ordered sort, after the scatter: the listed blocks' cell counters (offsets by now)
+// back to zero for the next sort; the other blocks' counters were never touched.
+template <int DIM>
+__global__ __launch_bounds__(64 << DIM) void k_cell_clear(const PBlk *plist, const int *npblk, unsigned *fcnt) {
+    const int nb = *npblk;
+    for (int e = blockIdx.x; e < nb; e += gridDim.x) fcnt[(long long)plist[e].key * (64 << DIM) + threadIdx.x] = 0u;
 }
 
 __global__ void k_clear_u64(unsigned long long *p, int n) {
@@ -207,14 +218,14 @@ template <class E, int NF>
 __global__ __launch_bounds__(256) void k_bin_scatter(long long np, const E *P, E *P2, const int8_t *act,
                                                      int8_t *act2, const int *pid, int *pid2, const int *pkey,
                                                      const int *prank, const unsigned *off, int inactive_key,
-                                                     const unsigned *inact_off, const unsigned *foff,
+                                                     const unsigned *inact_off, const unsigned *coff,
                                                      const uint8_t *pcell) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     const int key = pkey[p];
     long long q = (long long)off[key] + prank[p];
     if (key == inactive_key) q += inact_off[blockIdx.x];  // same 256-slot blocks as k_bin_count
-    else if (foff) q += foff[(long long)key * 64 + pcell[p]];  // cell-ordered: the cell's start
+    else if (coff) q += coff[(long long)key * 64 + pcell[p]];  // cell-ordered: the cell's start
     const int o = pid[p];
     if (o < 0) {  // unused capacity: nothing to move
         act2[q] = 0;
