@@ -45,9 +45,10 @@ __device__ __forceinline__ int body_of_block(const KArgs &A, long long key) {
 // a few dozen points each) count with wavefront-aggregated atomics. The inactive bucket
 // can hold most of the slots (unused capacity of a slot pool), so it is ranked without
 // atomics: the rank inside the workgroup here, the workgroup's count in inact_count
-// (scanned into per-workgroup offsets before the scatter).
+// (scanned into per-workgroup offsets before the scatter). fcnt (cell-ordered sort, else
+// null): 64 counters per (block, octant) key, all zero on entry; an active slot's rank is
+// then its rank inside its dual cell of the bucket, and pcell its cell.
 template <int DIM, class PT>
-// fcnt (cell-ordered sort, else null): 64 counters per (block, octant) key, all zero on entry
 __global__ __launch_bounds__(256) void k_bin_count(KArgs A, unsigned *count, int *pkey, int *prank,
                                                    long long inactive_key, unsigned *inact_count, unsigned *fcnt,
                                                    uint8_t *pcell) {
